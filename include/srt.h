@@ -1,0 +1,140 @@
+/*
+ * srt.h -- C ABI of the MI355X snapshot ray tracer (libsrt.so).
+ *
+ * The reference (JulianAdamek/snapshot-raytracer) has no library API: its hot
+ * path is the CUDA kernel
+ *     raytracing_kernel(const float* data, const __half* B1, B2, B3,
+ *                       const int* dims, int64_t Nside, int64_t iend,
+ *                       int n_steps, double* theta, double* phi,
+ *                       float* dA, eps1, eps2, rotation, defl1, defl2)
+ * (snapshot-raytracer.cu:552-555), fed by main() (.cu:600-998), which owns
+ * every device buffer, uploads the potential and the shift (.cu:721-855),
+ * computes the RING pixel angles on the host (.cu:857-892), launches the
+ * kernel (.cu:897-913) and copies the six maps back (.cu:922-987).  The entry
+ * points below replace those steps one for one; each names the reference
+ * lines it stands in for.
+ *
+ * Conventions: every function returns an srt_status (0 = success) and never
+ * exits the process.  Calls on one context are serialised by the caller;
+ * different contexts are independent.  Grids are x-fastest:
+ * value(ix, iy, iz) at ix + nx * (iy + ny * iz), which is HDF5's C order with
+ * x the last HDF5 axis (.cu:220, 253-256).
+ */
+#ifndef SRT_H
+#define SRT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct srt_context srt_context;
+
+typedef enum {
+    SRT_OK = 0,
+    SRT_ERR_INVALID_ARGUMENT = 1, /* bad size, index, enum or null pointer */
+    SRT_ERR_DEVICE = 2,           /* a HIP runtime call failed */
+    SRT_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed */
+    SRT_ERR_NOT_UPLOADED = 4,     /* srt_trace before srt_upload */
+    SRT_ERR_SHAPE = 5,            /* non-cubic or inconsistent grid dimensions */
+    SRT_ERR_UNSUPPORTED = 6       /* valid request this build cannot serve */
+} srt_status;
+
+/* Arithmetic of the ray kernel. */
+typedef enum {
+    /* Everything in FP64: interpolation, right-hand side, state. */
+    SRT_PREC_FP64 = 0,
+    /* Interpolation and right-hand side (geodesic, Jacobi map, shear) in FP32
+     * with FP32 accumulation; the 19-component state and the RK4 combination
+     * in FP64, with compensated summation for the position and dA. */
+    SRT_PREC_MIXED = 1,
+    /* The reference GPU code's arithmetic: FP32 interpolation with the cell
+     * offset taken as float(x) - float(floor(x)) (.cu:228-230), FP32
+     * shift-coupling intermediates (.cu:366-396), FP64 state (.cu:513-549). */
+    SRT_PREC_REF = 2
+} srt_precision;
+
+/* Device storage of the shift vector field. */
+typedef enum {
+    SRT_SHIFT_F32 = 0,       /* float, as read from the file */
+    SRT_SHIFT_F16_SCALED = 1 /* binary16 of B * 2^27, the reference's storage (.cu:585-597, 241-244) */
+} srt_shift_format;
+
+/* Order in which rays are assigned to GPU lanes (outputs always land in the
+ * caller's slot order). */
+typedef enum {
+    SRT_ORDER_LINEAR = 0, /* lane i traces slot i */
+    SRT_ORDER_NESTED = 1  /* full-sky traces only: lanes walk NESTED pixel order (compact sky
+                             patches per wavefront), results are written to RING slots */
+} srt_ray_order;
+
+/* Replaces the device selection and allocation bookkeeping of main()
+ * (.cu:690-772).  device = HIP device ordinal. */
+int srt_create(int device, srt_context** ctx);
+
+/* Frees every device buffer of the context (.cu:917-920, 989-998). */
+int srt_destroy(srt_context* ctx);
+
+/* Static text for a status code. */
+const char* srt_strerror(int status);
+
+/* Text of the last failure on the calling thread (includes HIP error names). */
+const char* srt_last_error(void);
+
+/* Uploads one snapshot (.cu:738-855: the potential, and the shift converted to
+ * its device format by vector3_to_half, .cu:585-597).
+ *   phi:       nx*ny*nz floats (the potential, HDF5 dataset /data, .cu:640)
+ *   shift_xyz: nx*ny*nz*3 floats, interleaved (B_x, B_y, B_z) per voxel
+ *              (HDF5 /data of H5T_ARRAY{3} float, .cu:676, 89-186)
+ *   inputs_on_device: 0 = host pointers, 1 = device pointers of this context's device
+ *   shift_format: srt_shift_format
+ * The grid must be cubic (nx == ny == nz, 4 <= nx <= 2048); the reference
+ * silently assumes equal dimensions for both files (.cu:645, 676, 752-772).
+ * Replaces any previous snapshot of the context. */
+int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64_t nx, int64_t ny, int64_t nz,
+               int inputs_on_device, int shift_format);
+
+/* Traces rays and writes the six maps (raytracing_kernel, .cu:552-583, with
+ * the pixel angles of .cu:857-892 computed on the device).
+ *   Ray slot i traces RING pixel pix_list[i] if pix_list is not null (device
+ *   pointer, count int64 values), else pixel pix_begin + i (.cu:708-717).
+ *   observer: 3 doubles in grid units (host pointer); null = (nx, ny, nz) / 2 (.cu:560).
+ *   out: device pointer, 6 * count floats, map-major: out[m * count + i] with
+ *        m = dA, ellipticity1, ellipticity2, rotation, deflection1, deflection2
+ *        (.cu:577-582).
+ *   final_state: optional device pointer, count * 19 doubles: the state after
+ *        the last step (x[3], n[3], e1[3], e2[3], dA, dA', sigma1, sigma2,
+ *        eps1, eps2, rotation) in the precision of the kernel.
+ *   stream: hipStream_t to launch on (null = the default stream).  The call is
+ *        asynchronous with respect to the host.
+ *   ray_order: srt_ray_order.  SRT_ORDER_NESTED needs pix_list == null,
+ *        pix_begin == 0 and count == 12 * nside^2. */
+int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+              int n_steps, const double* observer, int precision, int ray_order, float* out,
+              double* final_state, void* stream);
+
+/* Synchronous host-memory variant of srt_trace for callers without device
+ * buffers (the CLI): pix_list, out (6 * count floats) and final_state are host
+ * pointers; pix_list and final_state may be null. */
+int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+                   int n_steps, const double* observer, int precision, int ray_order, float* out,
+                   double* final_state);
+
+/* Milliseconds between HIP events recorded on the launch stream just before
+ * and just after the kernel launches of the last srt_trace call of this
+ * context (blocks until that work is done). */
+int srt_last_trace_ms(srt_context* ctx, float* ms);
+
+/* Information about the uploaded snapshot: dims[3] and padded edge length of
+ * the device grids. */
+int srt_grid_info(srt_context* ctx, int64_t* nx, int64_t* ny, int64_t* nz, int64_t* padded_edge);
+
+/* Version string of the library. */
+const char* srt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRT_H */

@@ -1,0 +1,201 @@
+"""MI355X snapshot ray tracer: Python binding of libsrt.so (include/srt.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  This
+module binds it with ctypes and mirrors the reference's host flow
+(snapshot-raytracer.cu:600-998): upload a snapshot, trace the rays of a RING
+batch (or an explicit pixel list), get the six maps.  There is no CPU
+fallback: every call goes to the HIP library and fails loudly if it is not
+built or no GPU is present.
+
+Import name: ``snapshot_raytracer_amd`` (the directory is
+``snapshot-raytracer_amd``; ``load_package()`` in the repository root
+modules registers it).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsrt.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+PREC_REF = 2
+PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "ref": PREC_REF}
+
+SHIFT_F32 = 0
+SHIFT_F16_SCALED = 1
+
+ORDER_LINEAR = 0
+ORDER_NESTED = 1
+
+MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "deflection1map", "deflection2map")
+
+# C-ABI functions declared in include/srt.h
+EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_trace",
+           "srt_trace_host", "srt_last_trace_ms", "srt_grid_info", "srt_version")
+
+_lib = None
+
+
+class SrtError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: status {status}")
+
+
+def lib():
+    """Load libsrt.so (built by `make -C snapshot-raytracer_amd`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                               "or make -C snapshot-raytracer_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        i32 = ctypes.c_int
+        L.srt_create.argtypes = [i32, ctypes.POINTER(P)]
+        L.srt_destroy.argtypes = [P]
+        L.srt_strerror.argtypes = [i32]
+        L.srt_strerror.restype = ctypes.c_char_p
+        L.srt_last_error.restype = ctypes.c_char_p
+        L.srt_version.restype = ctypes.c_char_p
+        L.srt_upload.argtypes = [P, P, P, i64, i64, i64, i32, i32]
+        L.srt_trace.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P, P]
+        L.srt_trace_host.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P]
+        L.srt_last_trace_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+        L.srt_grid_info.argtypes = [P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        L = lib()
+        msg = L.srt_last_error().decode(errors="replace")
+        raise SrtError(rc, f"{what} failed: {L.srt_strerror(rc).decode()} ({msg})")
+
+
+def batch_range(nside: int, batch: int | None) -> tuple[int, int]:
+    """(first RING pixel, pixel count) of a batch, as .cu:708-717 defines it.
+
+    batch None (or < 0) is the full sky.  Batch b covers RING pixels
+    [b Nside^2, (b + 1) Nside^2): a range of rings, not a HEALPix base pixel.
+    """
+    npix = 12 * nside * nside
+    if batch is None or batch < 0:
+        return 0, npix
+    start = batch * nside * nside
+    if start >= npix:
+        raise ValueError("Batch exceeds total number of pixels")
+    return start, nside * nside
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    """Address of a numpy array or torch tensor (host or device)."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Tracer:
+    """One library context on one GPU holding one uploaded snapshot."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = ctypes.c_void_p()
+        _check(lib().srt_create(device, ctypes.byref(self._ctx)), "srt_create")
+        self.device = device
+        self.n = None
+
+    def close(self) -> None:
+        if self._ctx:
+            lib().srt_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, phi, shift_xyz, shift_format: int = SHIFT_F32) -> None:
+        """phi: (n, n, n) float32; shift_xyz: (n, n, n, 3) float32 interleaved.
+
+        numpy arrays are host inputs; torch tensors on the GPU are device inputs.
+        """
+        on_dev = 0
+        if isinstance(phi, np.ndarray):
+            phi = np.ascontiguousarray(phi, dtype=np.float32)
+            shift_xyz = np.ascontiguousarray(shift_xyz, dtype=np.float32)
+        else:
+            on_dev = 1 if phi.is_cuda else 0
+            phi = phi.contiguous()
+            shift_xyz = shift_xyz.contiguous()
+            if not on_dev:
+                phi = phi.numpy()
+                shift_xyz = shift_xyz.numpy()
+        nz, ny, nx = phi.shape
+        if tuple(shift_xyz.shape) != (nz, ny, nx, 3):
+            raise ValueError("shift must be (nz, ny, nx, 3) interleaved")
+        _check(lib().srt_upload(self._ctx, _ptr(phi), _ptr(shift_xyz), nx, ny, nz, on_dev, shift_format),
+               "srt_upload")
+        self.n = nx
+
+    def trace_device(self, nside: int, n_steps: int, out, *, pix_begin: int = 0, count: int | None = None,
+                     pix_list=None, observer=None, precision: int = PREC_FP64, order: int = ORDER_LINEAR,
+                     final_state=None, stream=None) -> None:
+        """Asynchronous trace into device buffers (torch tensors on the GPU).
+
+        out: float32 tensor of 6 * count; pix_list: int64 tensor or None;
+        final_state: float64 tensor of count * 19 or None; stream: torch stream.
+        """
+        if count is None:
+            count = pix_list.numel() if pix_list is not None else 12 * nside * nside - pix_begin
+        obs = None if observer is None else np.ascontiguousarray(observer, dtype=np.float64)
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _check(lib().srt_trace(self._ctx, nside, pix_begin, count, _ptr(pix_list), n_steps, _ptr(obs),
+                               precision, order, _ptr(out), _ptr(final_state), st), "srt_trace")
+
+    def last_trace_ms(self) -> float:
+        ms = ctypes.c_float()
+        _check(lib().srt_last_trace_ms(self._ctx, ctypes.byref(ms)), "srt_last_trace_ms")
+        return float(ms.value)
+
+    def trace(self, nside: int, n_steps: int, *, batch: int | None = None, pix=None, observer=None,
+              precision: int = PREC_FP64, order: int = ORDER_LINEAR, return_state: bool = False):
+        """Synchronous trace with host outputs.
+
+        Returns maps (6, count) float32 in slot order (RING pixel order of the
+        batch, or the order of `pix`), plus the final state (count, 19) when
+        return_state is set.
+        """
+        pix_arr = None
+        if pix is not None:
+            pix_arr = np.ascontiguousarray(pix, dtype=np.int64)
+            start, count = 0, pix_arr.size
+        else:
+            start, count = batch_range(nside, batch)
+        out = np.empty((6, count), np.float32)
+        state = np.empty((count, 19), np.float64) if return_state else None
+        obs = None if observer is None else np.ascontiguousarray(observer, dtype=np.float64)
+        _check(lib().srt_trace_host(self._ctx, nside, start, count, _ptr(pix_arr), n_steps, _ptr(obs),
+                                    precision, order, _ptr(out), _ptr(state)), "srt_trace_host")
+        return (out, state) if return_state else out
+
+
+def full_sky_maps(nside: int, batch: int | None, maps: np.ndarray) -> np.ndarray:
+    """Place batch maps (6, count) into zeroed full-sky RING maps (6, 12 nside^2), .cu:924-942."""
+    start, count = batch_range(nside, batch)
+    full = np.zeros((6, 12 * nside * nside), np.float32)
+    full[:, start:start + count] = maps
+    return full
+
+
+def version() -> str:
+    return lib().srt_version().decode()

@@ -1,0 +1,449 @@
+// Device-side building blocks of the ray kernel (gfx950).
+//
+//   healpix ring pixel -> direction ..... chealpix pix2ang_ring64 as called at
+//                                         snapshot-raytracer.cu:865 (moved on device)
+//   nested -> ring pixel index .......... used only to order lanes (NESTED tiles)
+//   Catmull-Rom weights ................. .cu:195-212, as weights w(t) on the 4 taps
+//   tricubic value/gradient/Hessian ..... .cu:215-312, fused x -> y -> z contraction
+//   right-hand side ..................... .cu:315-510
+//
+// Grid layout in HBM: every field (potential, B_x, B_y, B_z) is a padded cube
+// of edge P = n + 3, x fastest, holding value(i, j, k) at
+// (i + 1) + P * ((j + 1) + P * (k + 1)) for i, j, k in [-1, n + 1], with the
+// periodic images in the halo.  Every 4-tap row of the stencil is then 16
+// contiguous bytes and no index wraps inside the stencil: a position is wrapped
+// once per interpolation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace srt {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ------------------------------------------------------------- HEALPix ----
+__device__ __forceinline__ int64_t isqrt64(int64_t v) {
+    int64_t r = (int64_t)sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+// RING pixel p of resolution n -> z = cos(theta), s = sin(theta), phi
+// (Gorski et al. 2005, ApJ 622, 759, sec. 4).
+__device__ __forceinline__ void ring_pixel_direction(int64_t n, int64_t p, double& z, double& s, double& phi) {
+    const int64_t ncap = 2 * n * (n - 1);
+    const int64_t npix = 12 * n * n;
+    const double dn = (double)n;
+    if (p < ncap) {
+        const int64_t i = (1 + isqrt64(1 + 2 * p)) / 2;
+        const int64_t j = p + 1 - 2 * i * (i - 1);
+        const double t = (double)i * (double)i / (3.0 * dn * dn);
+        z = 1.0 - t;
+        s = sqrt(t * (2.0 - t));
+        phi = ((double)j - 0.5) * kPi / (2.0 * (double)i);
+    } else if (p < npix - ncap) {
+        const int64_t q = p - ncap;
+        const int64_t i = q / (4 * n) + n;
+        const int64_t j = q - (i - n) * (4 * n) + 1;
+        const double sh = ((i + n) & 1) ? 1.0 : 0.5;
+        z = (double)(2 * n - i) * 2.0 / (3.0 * dn);
+        s = sqrt((1.0 - z) * (1.0 + z));
+        phi = ((double)j - sh) * kPi / (2.0 * dn);
+    } else {
+        const int64_t q = npix - p;
+        const int64_t i = (1 + isqrt64(2 * q - 1)) / 2;
+        const int64_t j = 4 * i + 1 - (q - 2 * i * (i - 1));
+        const double t = (double)i * (double)i / (3.0 * dn * dn);
+        z = t - 1.0;
+        s = sqrt(t * (2.0 - t));
+        phi = ((double)j - 0.5) * kPi / (2.0 * (double)i);
+    }
+}
+
+// NESTED pixel -> RING pixel (face / in-face coordinates, Gorski et al. 2005).
+// Used to hand each wavefront a compact 8 x 8 patch of the sky.
+__device__ __forceinline__ uint32_t compact_even_bits(uint64_t v) {
+    v &= 0x5555555555555555ull;
+    v = (v | (v >> 1)) & 0x3333333333333333ull;
+    v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
+    v = (v | (v >> 16)) & 0x00000000ffffffffull;
+    return (uint32_t)v;
+}
+
+__device__ __forceinline__ int64_t nest_to_ring(int64_t n, int64_t p) {
+    const int64_t npface = n * n;
+    const int face = (int)(p / npface);
+    const uint64_t ipf = (uint64_t)(p - (int64_t)face * npface);
+    const int64_t ix = compact_even_bits(ipf);
+    const int64_t iy = compact_even_bits(ipf >> 1);
+    const int jrll = (face < 4) ? 2 : (face < 8 ? 3 : 4);
+    const int jpll = (face < 4) ? (2 * face + 1) : (face < 8 ? 2 * (face - 4) : 2 * (face - 8) + 1);
+    const int64_t jr = (int64_t)jrll * n - ix - iy - 1;
+    const int64_t npix = 12 * n * n, ncap = 2 * n * (n - 1);
+    int64_t nr, before, kshift;
+    if (jr < n) {
+        nr = jr;
+        before = 2 * nr * (nr - 1);
+        kshift = 0;
+    } else if (jr > 3 * n) {
+        nr = 4 * n - jr;
+        before = npix - 2 * (nr + 1) * nr;
+        kshift = 0;
+    } else {
+        nr = n;
+        before = ncap + (jr - n) * 4 * n;
+        kshift = (jr - n) & 1;
+    }
+    int64_t jp = ((int64_t)jpll * nr + ix - iy + 1 + kshift) / 2;
+    if (jp > 4 * n)
+        jp -= 4 * n;
+    else if (jp < 1)
+        jp += 4 * n;
+    return before + jp - 1;
+}
+
+// ---------------------------------------------------------- Catmull-Rom ----
+// For taps p0..p3 at offsets -1, 0, 1, 2 the Keys (a = -1/2) cubic is
+// value = sum w_i p_i, d/dt = sum d_i p_i, d2/dt2 = sum dd_i p_i with
+//   w  = 1/2 (-t + 2t^2 - t^3,  2 - 5t^2 + 3t^3,  t + 4t^2 - 3t^3,  -t^2 + t^3)
+//   d  = 1/2 (-1 + 4t - 3t^2,  -10t + 9t^2,       1 + 8t - 9t^2,   -2t + 3t^2)
+//   dd =     (2 - 3t,          -5 + 9t,           4 - 9t,          -1 + 3t)
+// (the same polynomials as .cu:195-212, regrouped by tap).
+template <class R>
+struct Axis {
+    R w[4], d[4], dd[4];
+};
+
+template <class R>
+__device__ __forceinline__ Axis<R> cr_axis(R t) {
+    const R t2 = t * t, t3 = t2 * t;
+    const R h = R(0.5);
+    Axis<R> a;
+    a.w[0] = h * (R(2) * t2 - t - t3);
+    a.w[1] = h * (R(2) - R(5) * t2 + R(3) * t3);
+    a.w[2] = h * (t + R(4) * t2 - R(3) * t3);
+    a.w[3] = h * (t3 - t2);
+    a.d[0] = h * (R(4) * t - R(1) - R(3) * t2);
+    a.d[1] = h * (R(9) * t2 - R(10) * t);
+    a.d[2] = h * (R(1) + R(8) * t - R(9) * t2);
+    a.d[3] = h * (R(3) * t2 - R(2) * t);
+    a.dd[0] = R(2) - R(3) * t;
+    a.dd[1] = R(9) * t - R(5);
+    a.dd[2] = R(4) - R(9) * t;
+    a.dd[3] = R(3) * t - R(1);
+    return a;
+}
+
+// ---------------------------------------------------------------- grids ----
+struct Grids {
+    const float* phi;   // potential, sampled at x
+    const float* b[3];  // shift components; b[a] sampled at x - e_a / 2 (.cu:377, 406, 432)
+    int n;              // cube edge
+    int pad;            // padded edge n + 3
+    double inv_n;
+};
+
+// Wraps floor(x) into [0, n) and returns the offset x - floor(x) (exact in FP64).
+__device__ __forceinline__ int locate(double x, int n, double inv_n, double& t) {
+    const double fl = floor(x);
+    t = x - fl;
+    const double q = floor(fl * inv_n);
+    // fl and q * n are integers below 2^53, so r is exact; one correction
+    // covers the rounding of fl * inv_n
+    int r = (int)(fl - q * (double)n);
+    r += (r < 0) ? n : 0;
+    r -= (r >= n) ? n : 0;
+    return r;
+}
+
+struct alignas(4) Row4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ Row4 load_row(const float* p) {
+    Row4 r;
+    __builtin_memcpy(&r, p, sizeof(Row4));
+    return r;
+}
+
+// value, gradient (x, y, z), Hessian (xx, xy, xz, yy, yz, zz)
+template <class R>
+struct Interp {
+    R v, gx, gy, gz, hxx, hxy, hxz, hyy, hyz, hzz;
+};
+
+// Tricubic interpolation as one fused contraction: 16 rows x 3 (x-pass),
+// 16 x 6 (y-pass), 4 x 10 (z-pass) = 328 multiply-adds.  `base` points at
+// tap (-1, -1, -1) of the stencil in the padded grid.
+template <class R>
+__device__ __forceinline__ Interp<R> tricubic(const float* __restrict__ base, size_t pitch_y, size_t pitch_z,
+                                              const Axis<R>& ax, const Axis<R>& ay, const Axis<R>& az) {
+    Interp<R> o;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        R yv, yd, ydd, xd, xdyd, xdd;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const Row4 q = load_row(base + b * pitch_y + c * pitch_z);
+            const R p0 = R(q.x), p1 = R(q.y), p2 = R(q.z), p3 = R(q.w);
+            const R v = ax.w[0] * p0 + ax.w[1] * p1 + ax.w[2] * p2 + ax.w[3] * p3;
+            const R d = ax.d[0] * p0 + ax.d[1] * p1 + ax.d[2] * p2 + ax.d[3] * p3;
+            const R dd = ax.dd[0] * p0 + ax.dd[1] * p1 + ax.dd[2] * p2 + ax.dd[3] * p3;
+            if (b == 0) {
+                yv = ay.w[0] * v;
+                yd = ay.d[0] * v;
+                ydd = ay.dd[0] * v;
+                xd = ay.w[0] * d;
+                xdyd = ay.d[0] * d;
+                xdd = ay.w[0] * dd;
+            } else {
+                yv += ay.w[b] * v;
+                yd += ay.d[b] * v;
+                ydd += ay.dd[b] * v;
+                xd += ay.w[b] * d;
+                xdyd += ay.d[b] * d;
+                xdd += ay.w[b] * dd;
+            }
+        }
+        if (c == 0) {
+            o.v = az.w[0] * yv;
+            o.gz = az.d[0] * yv;
+            o.hzz = az.dd[0] * yv;
+            o.gy = az.w[0] * yd;
+            o.hyz = az.d[0] * yd;
+            o.hyy = az.w[0] * ydd;
+            o.gx = az.w[0] * xd;
+            o.hxz = az.d[0] * xd;
+            o.hxy = az.w[0] * xdyd;
+            o.hxx = az.w[0] * xdd;
+        } else {
+            o.v += az.w[c] * yv;
+            o.gz += az.d[c] * yv;
+            o.hzz += az.dd[c] * yv;
+            o.gy += az.w[c] * yd;
+            o.hyz += az.d[c] * yd;
+            o.hyy += az.w[c] * ydd;
+            o.gx += az.w[c] * xd;
+            o.hxz += az.d[c] * xd;
+            o.hxy += az.w[c] * xdyd;
+            o.hxx += az.w[c] * xdd;
+        }
+    }
+    return o;
+}
+
+// ------------------------------------------------------------------ RHS ----
+// Precision policy of one kernel form.
+//   RI: interpolation arithmetic and the type the shift-coupling
+//       intermediates are rounded to; RA: right-hand-side assembly and the
+//       derivative type; kRefOffset: cell offset taken as float(x) - float(i)
+//       (.cu:228-230) instead of the exact x - i.
+template <class RI_, class RA_, bool kRefOffset_>
+struct Form {
+    using RI = RI_;
+    using RA = RA_;
+    static constexpr bool kRefOffset = kRefOffset_;
+};
+using FormFP64 = Form<double, double, false>;
+using FormMixed = Form<float, float, false>;
+using FormRef = Form<float, double, true>;
+
+template <class F>
+__device__ __forceinline__ typename F::RI cell_offset(double x, double t) {
+    using RI = typename F::RI;
+    if constexpr (F::kRefOffset) {
+        return (RI)((float)x - (float)(x - t));  // float(x) - float(floor(x))
+    } else {
+        return (RI)t;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R rnorm3(R a, R b, R c) {
+    if constexpr (sizeof(R) == 4) {
+        return __frsqrt_rn(a * a + b * b + c * c);
+    } else {
+        return R(1) / sqrt(a * a + b * b + c * c);
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R sqrt_r(R a) {
+    return sqrt(a);
+}
+
+// Stage state in FP64: position x[3], direction nd[3] (unnormalised), D, D'.
+// ys = (e1[3], e2[3], sigma1, sigma2, eps1, eps2) in RA (the rotation angle is
+// never read).  Outputs: kx = x' in FP64, assembled as -nd * (|nd|^-1 (1 + 2 phi))
+// - B so the transverse direction keeps FP64 accuracy; dy[3..18] the other
+// derivatives in RA (dy[0..2] repeat x', dy[12] = -D').  The dA > 1/2 branch
+// (.cu:497-509) is decided on the FP64 D.
+template <class F>
+__device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x, const double* __restrict__ nd,
+                                    double Dd, double Dpd, const typename F::RA* __restrict__ ys,
+                                    double* __restrict__ kx, typename F::RA* __restrict__ dy) {
+    using RI = typename F::RI;
+    using RA = typename F::RA;
+    // unit direction and screen basis (the state itself is not renormalised)
+    RA n[3], e1[3], e2[3];
+    {
+        const RA nr[3] = {(RA)nd[0], (RA)nd[1], (RA)nd[2]};
+        const RA rn = rnorm3<RA>(nr[0], nr[1], nr[2]);
+        const RA r1 = rnorm3<RA>(ys[0], ys[1], ys[2]);
+        const RA r2 = rnorm3<RA>(ys[3], ys[4], ys[5]);
+        for (int i = 0; i < 3; ++i) {
+            n[i] = nr[i] * rn;
+            e1[i] = ys[i] * r1;
+            e2[i] = ys[3 + i] * r2;
+        }
+    }
+    const RA D = (RA)Dd, Dp = (RA)Dpd, s1 = ys[6], s2 = ys[7], eps1 = ys[8], eps2 = ys[9];
+
+    // cells: c0 at x, c1 at x - 1/2 (staggered shift component along its own axis)
+    int i0[3], i1[3];
+    RI t0[3], t1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t;
+        i0[a] = locate(x[a], G.n, G.inv_n, t);
+        t0[a] = cell_offset<F>(x[a], t);
+        const double xs = x[a] - 0.5;
+        i1[a] = locate(xs, G.n, G.inv_n, t);
+        t1[a] = cell_offset<F>(xs, t);
+    }
+    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
+
+    // shift: N_j = sum_a n_a dB_a/dx_j, E = 1/2 sum_a [e_a (n.gB_a) + n_a (e.gB_a)],
+    // T_ij = sum_a n_a H^a_ij - 1/2 [(n H^i)_j + (n H^j)_i], L = sum_a n_a tr H^a
+    RI N[3] = {RI(0), RI(0), RI(0)};
+    RI T[6] = {RI(0), RI(0), RI(0), RI(0), RI(0), RI(0)};  // xx xy xz yy yz zz
+    RI L = RI(0), E1 = RI(0), E2 = RI(0);
+    RA Bv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int ix = (a == 0) ? i1[0] : i0[0];
+        const int iy = (a == 1) ? i1[1] : i0[1];
+        const int iz = (a == 2) ? i1[2] : i0[2];
+        const Axis<RI> ax = cr_axis<RI>(a == 0 ? t1[0] : t0[0]);
+        const Axis<RI> ay = cr_axis<RI>(a == 1 ? t1[1] : t0[1]);
+        const Axis<RI> az = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
+        const float* base = G.b[a] + (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
+        const Interp<RI> r = tricubic<RI>(base, py, pz, ax, ay, az);
+        Bv[a] = (RA)r.v;
+        const RA g[3] = {(RA)r.gx, (RA)r.gy, (RA)r.gz};
+        const RA H[3][3] = {{(RA)r.hxx, (RA)r.hxy, (RA)r.hxz},
+                            {(RA)r.hxy, (RA)r.hyy, (RA)r.hyz},
+                            {(RA)r.hxz, (RA)r.hyz, (RA)r.hzz}};
+        const RA ng = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
+        const RA e1g = e1[0] * g[0] + e1[1] * g[1] + e1[2] * g[2];
+        const RA e2g = e2[0] * g[0] + e2[1] * g[1] + e2[2] * g[2];
+        for (int j = 0; j < 3; ++j) N[j] = (RI)((RA)N[j] + n[a] * g[j]);
+        E1 = (RI)((RA)E1 + RA(0.5) * (e1[a] * ng + n[a] * e1g));
+        E2 = (RI)((RA)E2 + RA(0.5) * (e2[a] * ng + n[a] * e2g));
+        RA nH[3];
+        for (int j = 0; j < 3; ++j) nH[j] = n[0] * H[0][j] + n[1] * H[1][j] + n[2] * H[2][j];
+        int k = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = i; j < 3; ++j, ++k) {
+                RA c = n[a] * H[i][j];
+                if (i == a) c -= RA(0.5) * nH[j];
+                if (j == a) c -= RA(0.5) * nH[i];
+                T[k] = (RI)((RA)T[k] + c);
+            }
+        L = (RI)((RA)L + n[a] * (H[0][0] + H[1][1] + H[2][2]));
+    }
+
+    // potential at x (.cu:462)
+    RA phi, g[3], H[3][3];
+    {
+        const Axis<RI> ax = cr_axis<RI>(t0[0]);
+        const Axis<RI> ay = cr_axis<RI>(t0[1]);
+        const Axis<RI> az = cr_axis<RI>(t0[2]);
+        const float* base = G.phi + (size_t)i0[0] + (size_t)i0[1] * py + (size_t)i0[2] * pz;
+        const Interp<RI> r = tricubic<RI>(base, py, pz, ax, ay, az);
+        phi = (RA)r.v;
+        g[0] = (RA)r.gx;
+        g[1] = (RA)r.gy;
+        g[2] = (RA)r.gz;
+        H[0][0] = (RA)r.hxx;
+        H[0][1] = H[1][0] = (RA)r.hxy;
+        H[0][2] = H[2][0] = (RA)r.hxz;
+        H[1][1] = (RA)r.hyy;
+        H[1][2] = H[2][1] = (RA)r.hyz;
+        H[2][2] = (RA)r.hzz;
+    }
+
+    // x' = -n (1 + 2 phi) - B, with the norm of n in FP64: the position decides
+    // the stencil cell, and the Catmull-Rom Hessian jumps across cell faces, so
+    // the FP64 and mixed forms must track the same position to ~1e-10 cells
+    {
+        const double nn = nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2];
+        const double scale = (1.0 + 2.0 * (double)phi) / sqrt(nn);
+        for (int i = 0; i < 3; ++i) {
+            kx[i] = -nd[i] * scale - (double)Bv[i];
+            dy[i] = (RA)kx[i];
+        }
+    }
+    // n' = -P v with P = n n^T - 1 and v = 2 grad phi + N:  n' = v - n (n . v)
+    {
+        const RA v[3] = {RA(2) * g[0] + (RA)N[0], RA(2) * g[1] + (RA)N[1], RA(2) * g[2] + (RA)N[2]};
+        const RA nv = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+        for (int i = 0; i < 3; ++i) dy[3 + i] = v[i] - n[i] * nv;
+    }
+    // e' = -n [2 (e . grad phi) + E]
+    {
+        const RA c1 = RA(2) * (e1[0] * g[0] + e1[1] * g[1] + e1[2] * g[2]) + (RA)E1;
+        const RA c2 = RA(2) * (e2[0] * g[0] + e2[1] * g[1] + e2[2] * g[2]) + (RA)E2;
+        for (int i = 0; i < 3; ++i) {
+            dy[6 + i] = -n[i] * c1;
+            dy[9 + i] = -n[i] * c2;
+        }
+    }
+    // Jacobi map: D'' = -4 D' Gamma - (P : H) D + L D / 2, Gamma = n.grad phi + n.N / 4
+    dy[12] = -Dp;
+    const RA gam = n[0] * g[0] + n[1] * g[1] + n[2] * g[2] +
+                   RA(0.25) * (n[0] * (RA)N[0] + n[1] * (RA)N[1] + n[2] * (RA)N[2]);
+    RA nHn = RA(0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) nHn += n[i] * H[i][j] * n[j];
+    const RA PH = nHn - (H[0][0] + H[1][1] + H[2][2]);
+    RA d13 = RA(-4) * Dp * gam - PH * D + RA(0.5) * (RA)L * D;
+    // shear source uses H + T/2 (.cu:484-492)
+    RA Hm[3][3];
+    {
+        int k = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = i; j < 3; ++j, ++k) {
+                Hm[i][j] = (RA)(RI)(H[i][j] + RA(0.5) * (RA)T[k]);
+                Hm[j][i] = Hm[i][j];
+            }
+    }
+    RA S1H = RA(0), S2H = RA(0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            S1H += (e1[i] * e1[j] - e2[i] * e2[j]) * Hm[i][j];
+            S2H += (e1[i] * e2[j] + e1[j] * e2[i]) * Hm[i][j];
+        }
+    const RA D2 = D * D;
+    dy[14] = RA(4) * s1 * gam + S1H * D2;
+    dy[15] = RA(4) * s2 * gam + S2H * D2;
+    if (Dd > 0.5) {
+        const RA q = sqrt_r<RA>(RA(4) + eps1 * eps1 + eps2 * eps2);
+        const RA inv_D2 = RA(1) / D2;
+        d13 += (s1 * s1 + s2 * s2) * inv_D2 / D;
+        dy[16] = RA(-2) * s1 * q * inv_D2;
+        dy[17] = RA(-2) * s2 * q * inv_D2;
+        dy[18] = -(s1 * eps2 - s2 * eps1) * inv_D2 / (RA(2) + q);
+    } else {
+        dy[16] = RA(0);
+        dy[17] = RA(0);
+        dy[18] = RA(0);
+    }
+    dy[13] = d13;
+}
+
+}  // namespace srt

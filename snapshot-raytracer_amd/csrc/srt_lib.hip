@@ -1,0 +1,474 @@
+// libsrt.so: HIP kernels of the snapshot ray tracer for MI355X (gfx950) and
+// the C ABI declared in include/srt.h.
+//
+// Kernels
+//   pad_grid_kernel   dense n^3 (scalar or one component of an interleaved
+//                     vector field) -> padded (n+3)^3 periodic cube; optional
+//                     binary16(B * 2^27) quantisation, the reference's shift
+//                     storage (.cu:585-597, read back at .cu:241-244).
+//   trace_kernel<F>   one ray per lane: device HEALPix direction, RK4 over
+//                     n_steps with 4 right-hand-side evaluations per step,
+//                     each of 4 tricubic interpolations, then the six maps
+//                     (.cu:552-583).  F selects FP64, mixed or reference
+//                     arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/srt.h"
+#include "srt_device.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define SRT_HIP(call)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,                 \
+                        std::string(#call) + ": " + hipGetErrorName(e_) + " (" + hipGetErrorString(e_) + ")"); \
+        }                                                                                                   \
+    } while (0)
+
+// ------------------------------------------------------------- binary16 ----
+// IEEE-754 binary16, round to nearest even, from the format definition.
+__device__ __forceinline__ uint32_t f32_to_f16_bits(float f) {
+    const uint32_t x = __float_as_uint(f);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return ax == 0x7f800000u ? (sign | 0x7c00u) : (sign | 0x7e00u | ((ax >> 13) & 0x3ffu));
+    if (ax >= 0x477ff000u) return sign | 0x7c00u;
+    if (ax >= 0x38800000u) {
+        uint32_t h = (((ax >> 23) - 112u) << 10) | ((ax & 0x7fffffu) >> 13);
+        const uint32_t rem = ax & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+        return sign | h;
+    }
+    if (ax < 0x33000000u) return sign;
+    const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - (ax >> 23);
+    uint32_t h = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (h & 1u))) ++h;
+    return sign | h;
+}
+
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu;
+    const uint32_t m = h & 0x3ffu;
+    if (e == 0) return __uint_as_float(sign | __float_as_uint((float)m * 5.9604644775390625e-08f));
+    if (e == 31) return __uint_as_float(sign | 0x7f800000u | (m << 13));
+    return __uint_as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+// padded(i, j, k) = src(i - 1 mod n, j - 1 mod n, k - 1 mod n), i, j, k in [0, P)
+__global__ __launch_bounds__(256) void pad_grid_kernel(const float* __restrict__ src, int64_t stride, int comp,
+                                                       int n, int P, int quantise, float* __restrict__ dst) {
+    const int64_t total = (int64_t)P * P * P;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % P);
+        const int j = (int)((q / P) % P);
+        const int k = (int)(q / ((int64_t)P * P));
+        const int si = (i + n - 1) % n, sj = (j + n - 1) % n, sk = (k + n - 1) % n;
+        const int64_t s = (int64_t)si + (int64_t)n * ((int64_t)sj + (int64_t)n * sk);
+        float v = src[s * stride + comp];
+        if (quantise) v = f16_bits_to_f32(f32_to_f16_bits(v * 134217728.0f)) / 134217728.0f;
+        dst[q] = v;
+    }
+}
+
+// ---------------------------------------------------------------- trace ----
+struct TraceArgs {
+    srt::Grids G;
+    int64_t nside;
+    int64_t pix_begin;
+    int64_t count;
+    const int64_t* pix_list;
+    int order;
+    int n_steps;
+    double obs[3];
+    float* out;
+    double* state;
+};
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
+    using RA = typename F::RA;
+    const int64_t lane_ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (lane_ray >= A.count) return;
+    int64_t pix, slot;
+    if (A.order == SRT_ORDER_NESTED) {
+        pix = srt::nest_to_ring(A.nside, lane_ray);
+        slot = pix;
+    } else {
+        slot = lane_ray;
+        pix = A.pix_list ? A.pix_list[lane_ray] : A.pix_begin + lane_ray;
+    }
+    if (pix < 0 || pix >= 12 * A.nside * A.nside) {
+        // invalid entry of a caller's pixel list: NaN maps in its slot (in NESTED
+        // order the slot is the pixel itself, so nothing can be written)
+        if (A.order != SRT_ORDER_NESTED) {
+            const float nan = __builtin_nanf("");
+            for (int m = 0; m < 6; ++m) A.out[m * A.count + slot] = nan;
+        }
+        return;
+    }
+    double z, s, ph;
+    srt::ring_pixel_direction(A.nside, pix, z, s, ph);
+    double sp, cp;
+    sincos(ph, &sp, &cp);
+
+    // state (.cu:560): x, n = -r, e1 = theta-hat, e2 = (sin phi, -cos phi, 0), D = 0, D' = -1
+    double y[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp, 0.0,
+                    0.0,      -1.0,     0.0,      0.0,     0.0,     0.0, 0.0};
+    // RK4 (.cu:513-550).  Position and dA increments stay in FP64 in every
+    // form: x' carries the transverse deflection and the FP64 dA decides the
+    // dA > 1/2 branch of the right-hand side exactly as the FP64 oracle does.
+    // Other derivatives and their stage sums are in RA.
+    double accx[3], kx[3], accD, kD;
+    RA acc[19], k[19];
+    double comp[4] = {0.0, 0.0, 0.0, 0.0};  // compensated sums of x and dA (mixed form)
+
+    // stage state y + c * k, then the right-hand side into (kxo, ko, kDo)
+    auto eval = [&](double c, const double* kxi, const RA* ki, double kDi, double* kxo, RA* ko, double& kDo) {
+        double xs[3], ns[3];
+        RA ys[10];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            xs[i] = y[i] + c * kxi[i];
+            ns[i] = y[3 + i] + c * (double)ki[3 + i];
+        }
+        const double Ds = y[12] + c * kDi;
+        const double Dps = y[13] + c * (double)ki[13];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) ys[j] = (RA)(y[6 + j] + c * (double)ki[6 + j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ys[6 + j] = (RA)(y[14 + j] + c * (double)ki[14 + j]);
+        srt::rhs<F>(A.G, xs, ns, Ds, Dps, ys, kxo, ko);
+        kDo = -Dps;
+    };
+
+    for (int step = 0; step < A.n_steps; ++step) {
+        {
+            const double zx[3] = {0.0, 0.0, 0.0};
+            RA zk[19];
+#pragma unroll
+            for (int j = 0; j < 19; ++j) zk[j] = RA(0);
+            eval(0.0, zx, zk, 0.0, accx, acc, accD);  // k1
+        }
+        eval(0.5, accx, acc, accD, kx, k, kD);  // k2
+        {
+            const double kx2[3] = {kx[0], kx[1], kx[2]};
+            RA k2[19];
+#pragma unroll
+            for (int j = 0; j < 19; ++j) k2[j] = k[j];
+            const double kD2 = kD;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) accx[i] += 2.0 * kx[i];
+#pragma unroll
+            for (int j = 0; j < 19; ++j) acc[j] += RA(2) * k[j];
+            accD += 2.0 * kD;
+            eval(0.5, kx2, k2, kD2, kx, k, kD);  // k3
+        }
+        {
+            const double kx3[3] = {kx[0], kx[1], kx[2]};
+            RA k3[19];
+#pragma unroll
+            for (int j = 0; j < 19; ++j) k3[j] = k[j];
+            const double kD3 = kD;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) accx[i] += 2.0 * kx[i];
+#pragma unroll
+            for (int j = 0; j < 19; ++j) acc[j] += RA(2) * k[j];
+            accD += 2.0 * kD;
+            eval(1.0, kx3, k3, kD3, kx, k, kD);  // k4
+        }
+        constexpr double sixth = 1.0 / 6.0;
+        if constexpr (sizeof(RA) == 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = q < 3 ? q : 12;
+                const double inc = (q < 3 ? (accx[q] + kx[q]) : (accD + kD)) * sixth - comp[q];
+                const double t = y[j] + inc;
+                comp[q] = (t - y[j]) - inc;
+                y[j] = t;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) y[i] += (accx[i] + kx[i]) * sixth;
+            y[12] += (accD + kD) * sixth;
+        }
+#pragma unroll
+        for (int j = 3; j < 19; ++j)
+            if (j != 12) y[j] += (double)(acc[j] + k[j]) * sixth;
+    }
+
+    // deflection: Delta = unit(x - observer) - r (.cu:566-575)
+    double d[3] = {y[0] - A.obs[0], y[1] - A.obs[1], y[2] - A.obs[2]};
+    const double rn = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    d[0] = d[0] * rn - s * cp;
+    d[1] = d[1] * rn - s * sp;
+    d[2] = d[2] * rn - z;
+    const int64_t C = A.count;
+    A.out[0 * C + slot] = (float)y[12];
+    A.out[1 * C + slot] = (float)y[16];
+    A.out[2 * C + slot] = (float)y[17];
+    A.out[3 * C + slot] = (float)y[18];
+    A.out[4 * C + slot] = (float)(d[0] * z * cp + d[1] * z * sp - d[2] * s);
+    A.out[5 * C + slot] = (float)(d[0] * sp - d[1] * cp);
+    if (A.state) {
+#pragma unroll
+        for (int j = 0; j < 19; ++j) A.state[slot * 19 + j] = y[j];
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI ----
+struct srt_context {
+    int device = 0;
+    int n = 0;
+    int pad = 0;
+    float* grids = nullptr;  // 4 padded cubes: phi, Bx, By, Bz
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timed = false;
+};
+
+extern "C" {
+
+const char* srt_version(void) { return "srt 0.1 (gfx950)"; }
+
+const char* srt_strerror(int status) {
+    switch (status) {
+        case SRT_OK: return "success";
+        case SRT_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case SRT_ERR_DEVICE: return "HIP device error";
+        case SRT_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case SRT_ERR_NOT_UPLOADED: return "no snapshot uploaded";
+        case SRT_ERR_SHAPE: return "unsupported grid shape";
+        case SRT_ERR_UNSUPPORTED: return "unsupported request";
+        default: return "unknown status";
+    }
+}
+
+const char* srt_last_error(void) { return g_last_error.c_str(); }
+
+int srt_create(int device, srt_context** ctx) {
+    if (!ctx) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_create: ctx is null");
+    *ctx = nullptr;
+    int ndev = 0;
+    SRT_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_create: device " + std::to_string(device) + " of " +
+                                                  std::to_string(ndev));
+    SRT_HIP(hipSetDevice(device));
+    srt_context* c = new srt_context();
+    c->device = device;
+    hipError_t e1 = hipEventCreate(&c->ev_start);
+    hipError_t e2 = hipEventCreate(&c->ev_stop);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        srt_destroy(c);
+        return fail(SRT_ERR_DEVICE, "srt_create: hipEventCreate failed");
+    }
+    *ctx = c;
+    return SRT_OK;
+}
+
+int srt_destroy(srt_context* ctx) {
+    if (!ctx) return SRT_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->grids) (void)hipFree(ctx->grids);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    delete ctx;
+    return SRT_OK;
+}
+
+int srt_grid_info(srt_context* ctx, int64_t* nx, int64_t* ny, int64_t* nz, int64_t* padded_edge) {
+    if (!ctx) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_grid_info: ctx is null");
+    if (!ctx->grids) return fail(SRT_ERR_NOT_UPLOADED, "srt_grid_info: no snapshot uploaded");
+    if (nx) *nx = ctx->n;
+    if (ny) *ny = ctx->n;
+    if (nz) *nz = ctx->n;
+    if (padded_edge) *padded_edge = ctx->pad;
+    return SRT_OK;
+}
+
+int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64_t nx, int64_t ny, int64_t nz,
+               int inputs_on_device, int shift_format) {
+    if (!ctx || !phi || !shift_xyz) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_upload: null pointer");
+    if (nx != ny || ny != nz)
+        return fail(SRT_ERR_SHAPE, "srt_upload: grid must be cubic (got " + std::to_string(nx) + " x " +
+                                       std::to_string(ny) + " x " + std::to_string(nz) + ")");
+    if (nx < 4 || nx > 2048) return fail(SRT_ERR_SHAPE, "srt_upload: edge must be in [4, 2048]");
+    if (shift_format != SRT_SHIFT_F32 && shift_format != SRT_SHIFT_F16_SCALED)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_upload: unknown shift format");
+    SRT_HIP(hipSetDevice(ctx->device));
+    const int n = (int)nx, P = n + 3;
+    const size_t nvox = (size_t)n * n * n, pvox = (size_t)P * P * P;
+    if (ctx->grids) {
+        SRT_HIP(hipFree(ctx->grids));
+        ctx->grids = nullptr;
+        ctx->n = ctx->pad = 0;
+    }
+    float* grids = nullptr;
+    SRT_HIP(hipMalloc(&grids, 4 * pvox * sizeof(float)));
+    const float* dphi = phi;
+    const float* dshift = shift_xyz;
+    float* staging = nullptr;
+    if (!inputs_on_device) {
+        hipError_t e = hipMalloc(&staging, 4 * nvox * sizeof(float));
+        if (e != hipSuccess) {
+            (void)hipFree(grids);
+            return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
+                        std::string("srt_upload: staging hipMalloc: ") + hipGetErrorName(e));
+        }
+        e = hipMemcpy(staging, phi, nvox * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(staging + nvox, shift_xyz, 3 * nvox * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(staging);
+            (void)hipFree(grids);
+            return fail(SRT_ERR_DEVICE, std::string("srt_upload: hipMemcpy: ") + hipGetErrorName(e));
+        }
+        dphi = staging;
+        dshift = staging + nvox;
+    }
+    const int blocks = (int)std::min<int64_t>(((int64_t)pvox + 255) / 256, 65536);
+    hipLaunchKernelGGL(pad_grid_kernel, dim3(blocks), dim3(256), 0, 0, dphi, (int64_t)1, 0, n, P, 0, grids);
+    for (int a = 0; a < 3; ++a)
+        hipLaunchKernelGGL(pad_grid_kernel, dim3(blocks), dim3(256), 0, 0, dshift, (int64_t)3, a, n, P,
+                           shift_format == SRT_SHIFT_F16_SCALED ? 1 : 0, grids + (size_t)(1 + a) * pvox);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (staging) (void)hipFree(staging);
+    if (e != hipSuccess) {
+        (void)hipFree(grids);
+        return fail(SRT_ERR_DEVICE, std::string("srt_upload: pad kernel: ") + hipGetErrorName(e));
+    }
+    ctx->grids = grids;
+    ctx->n = n;
+    ctx->pad = P;
+    return SRT_OK;
+}
+
+int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+              int n_steps, const double* observer, int precision, int ray_order, float* out, double* final_state,
+              void* stream) {
+    if (!ctx || !out) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: null context or output");
+    if (!ctx->grids) return fail(SRT_ERR_NOT_UPLOADED, "srt_trace: no snapshot uploaded");
+    if (nside <= 0 || nside > (1 << 20)) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: nside out of range");
+    if (n_steps < 0) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: n_steps < 0");
+    const int64_t npix = 12 * nside * nside;
+    if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: count out of range");
+    if (!pix_list && (pix_begin < 0 || pix_begin + count > npix))
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: pixel range outside the sky");
+    if (ray_order == SRT_ORDER_NESTED) {
+        if (pix_list || pix_begin != 0 || count != npix)
+            return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: NESTED lane order needs the full sky");
+    } else if (ray_order != SRT_ORDER_LINEAR) {
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown ray order");
+    }
+    SRT_HIP(hipSetDevice(ctx->device));
+    TraceArgs A;
+    A.G.phi = ctx->grids;
+    const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
+    for (int a = 0; a < 3; ++a) A.G.b[a] = ctx->grids + (size_t)(1 + a) * pvox;
+    A.G.n = ctx->n;
+    A.G.pad = ctx->pad;
+    A.G.inv_n = 1.0 / (double)ctx->n;
+    A.nside = nside;
+    A.pix_begin = pix_begin;
+    A.count = count;
+    A.pix_list = pix_list;
+    A.order = ray_order;
+    A.n_steps = n_steps;
+    for (int i = 0; i < 3; ++i) A.obs[i] = observer ? observer[i] : 0.5 * (double)ctx->n;
+    A.out = out;
+    A.state = final_state;
+    hipStream_t st = (hipStream_t)stream;
+    SRT_HIP(hipEventRecord(ctx->ev_start, st));
+    if (count > 0) {
+        const int64_t blocks = (count + kBlock - 1) / kBlock;
+        switch (precision) {
+            case SRT_PREC_FP64:
+                hipLaunchKernelGGL(trace_kernel<srt::FormFP64>, dim3((unsigned)blocks), dim3(kBlock), 0, st, A);
+                break;
+            case SRT_PREC_MIXED:
+                hipLaunchKernelGGL(trace_kernel<srt::FormMixed>, dim3((unsigned)blocks), dim3(kBlock), 0, st, A);
+                break;
+            case SRT_PREC_REF:
+                hipLaunchKernelGGL(trace_kernel<srt::FormRef>, dim3((unsigned)blocks), dim3(kBlock), 0, st, A);
+                break;
+            default:
+                return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
+        }
+        SRT_HIP(hipGetLastError());
+    }
+    SRT_HIP(hipEventRecord(ctx->ev_stop, st));
+    ctx->timed = true;
+    return SRT_OK;
+}
+
+int srt_last_trace_ms(srt_context* ctx, float* ms) {
+    if (!ctx || !ms) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_last_trace_ms: null pointer");
+    if (!ctx->timed) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_last_trace_ms: no trace recorded");
+    SRT_HIP(hipSetDevice(ctx->device));
+    SRT_HIP(hipEventSynchronize(ctx->ev_stop));
+    SRT_HIP(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
+    return SRT_OK;
+}
+
+int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+                   int n_steps, const double* observer, int precision, int ray_order, float* out,
+                   double* final_state) {
+    if (!ctx || !out) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_host: null context or output");
+    if (count < 0) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_host: count < 0");
+    SRT_HIP(hipSetDevice(ctx->device));
+    const size_t c = (size_t)count;
+    float* d_out = nullptr;
+    int64_t* d_pix = nullptr;
+    double* d_state = nullptr;
+    int rc = SRT_OK;
+    auto cleanup = [&]() {
+        if (d_out) (void)hipFree(d_out);
+        if (d_pix) (void)hipFree(d_pix);
+        if (d_state) (void)hipFree(d_state);
+    };
+    hipError_t e = hipMalloc(&d_out, std::max<size_t>(1, 6 * c) * sizeof(float));
+    if (e == hipSuccess && pix_list) e = hipMalloc(&d_pix, std::max<size_t>(1, c) * sizeof(int64_t));
+    if (e == hipSuccess && final_state) e = hipMalloc(&d_state, std::max<size_t>(1, 19 * c) * sizeof(double));
+    if (e == hipSuccess && pix_list && c) e = hipMemcpy(d_pix, pix_list, c * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
+                    std::string("srt_trace_host: ") + hipGetErrorName(e));
+    }
+    rc = srt_trace(ctx, nside, pix_begin, count, d_pix, n_steps, observer, precision, ray_order, d_out, d_state,
+                   nullptr);
+    if (rc == SRT_OK) {
+        e = hipDeviceSynchronize();
+        if (e == hipSuccess && c) e = hipMemcpy(out, d_out, 6 * c * sizeof(float), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && final_state && c)
+            e = hipMemcpy(final_state, d_state, 19 * c * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(SRT_ERR_DEVICE, std::string("srt_trace_host: ") + hipGetErrorName(e));
+    }
+    cleanup();
+    return rc;
+}
+
+}  // extern "C"
