@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--subset-frac", type=float, default=0.01)
     p.add_argument("--no-oracle", action="store_true", help="skip the CPU oracle leg (accuracy + cpu baseline)")
     p.add_argument("--cpu-threads", type=int, default=16)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="first run one device copy of 1 GiB (a known-size kernel that scales the "
+                        "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     return p.parse_args()
 
 
@@ -79,6 +82,14 @@ def main():
     pkg = __graft_entry__.load_package()
     fields = __import__(__graft_entry__.PKG_NAME + ".fields", fromlist=["fields"])
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 1 GiB read + 1 GiB written", file=sys.stderr, flush=True)
+        del src, dst
     n, nside, n_steps = args.grid, args.nside, args.n_steps
     rays = 12 * nside * nside
     t0 = time.time()
