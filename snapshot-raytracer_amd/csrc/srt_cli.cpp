@@ -1,0 +1,186 @@
+// snapshot-raytracer command line: the drop-in for the reference's program.
+//
+//   snapshot-raytracer <hdf5_file_potential> <hdf5_file_B> <Nside> <n_steps> [<batch>]
+//
+// Same arguments, validation, batch semantics, output files and exit codes as
+// snapshot-raytracer.cu:600-1006 (README.md:27-49):
+//   * reads dataset /data of both files (potential: scalar; shift: array[3] of
+//     float per voxel);
+//   * batch b traces RING pixels [b Nside^2, (b+1) Nside^2), no batch = full sky
+//     (.cu:708-717);
+//   * writes dAmap, ellipticity1map, ellipticity2map, rotationmap,
+//     deflection1map, deflection2map as <name>_batch<b>.fits in the working
+//     directory, full-sky RING maps with zeros outside the batch (.cu:924-987);
+//   * returns 0 on success and -1 (exit status 255) on handled errors.
+// Differences, all on invalid input: non-numeric Nside/n_steps/batch is an
+// error (-1) rather than an uncaught exception; grids must be cubic with equal
+// potential and shift dimensions (the reference assumes it, .cu:645, 676).
+//
+// Environment (optional): SRT_PRECISION = mixed (default) | fp64 | ref selects
+// the kernel arithmetic; SRT_SHIFT = f32 (default) | f16 selects the device
+// shift storage (f16 is the reference's binary16 x 2^27); SRT_DEVICE = HIP
+// device ordinal.
+#include <cerrno>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/srt.h"
+#include "srt_io.hpp"
+
+namespace {
+
+bool parse_int64(const char* s, int64_t& v) {
+    errno = 0;
+    char* end = nullptr;
+    const long long r = std::strtoll(s, &end, 10);
+    if (errno || end == s || *end != '\0') return false;
+    v = r;
+    return true;
+}
+
+const char* env_or(const char* name, const char* dflt) {
+    const char* v = std::getenv(name);
+    return (v && *v) ? v : dflt;
+}
+
+void usage(const char* prog) {
+    std::cerr << "Usage: " << prog << " <hdf5_file_potential> <hdf5_file_B> <Nside> <n_steps> [<batch>]\n"
+              << "  <hdf5_file_potential>: HDF5 file containing a snapshot of the Weyl potential\n"
+              << "  <hdf5_file_B>: HDF5 file containing a snapshot of the frame-dragging potential (shift)\n"
+              << "  <Nside>: HEALPix Nside parameter for output maps\n"
+              << "  <n_steps>: Number of integration steps for raytracing (1 step = 1 grid unit)\n"
+              << "  [<batch>]: Optional batch index for processing a subset of pixels" << std::endl;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+    if (argc < 5) {
+        usage(argv[0]);
+        return -1;
+    }
+    int64_t nside = 0, n_steps = 0, batch = -1;
+    if (!parse_int64(argv[3], nside) || nside <= 0) {
+        std::cerr << "Invalid Nside value: " << argv[3] << std::endl;
+        return -1;
+    }
+    if (!parse_int64(argv[4], n_steps) || n_steps <= 0 || n_steps > 0x7fffffff) {
+        std::cerr << "Invalid n_steps value: " << argv[4] << std::endl;
+        return -1;
+    }
+    if (argc > 5) {
+        if (!parse_int64(argv[5], batch) || batch < 0) {
+            std::cerr << "Invalid batch value: " << argv[5] << std::endl;
+            return -1;
+        }
+    }
+    const std::string prec_s = env_or("SRT_PRECISION", "mixed");
+    int precision = SRT_PREC_MIXED;
+    if (prec_s == "fp64")
+        precision = SRT_PREC_FP64;
+    else if (prec_s == "ref")
+        precision = SRT_PREC_REF;
+    else if (prec_s != "mixed") {
+        std::cerr << "Invalid SRT_PRECISION: " << prec_s << " (mixed, fp64 or ref)" << std::endl;
+        return -1;
+    }
+    const std::string shift_s = env_or("SRT_SHIFT", "f32");
+    if (shift_s != "f32" && shift_s != "f16") {
+        std::cerr << "Invalid SRT_SHIFT: " << shift_s << " (f32 or f16)" << std::endl;
+        return -1;
+    }
+    int64_t device = 0;
+    if (!parse_int64(env_or("SRT_DEVICE", "0"), device) || device < 0) {
+        std::cerr << "Invalid SRT_DEVICE" << std::endl;
+        return -1;
+    }
+
+    std::cout << "Nside: " << nside << std::endl;
+    std::cout << "n_steps: " << n_steps << std::endl;
+    std::cout << "Loading potential data from HDF5 file " << argv[1] << " ..." << std::endl;
+    std::string err;
+    std::vector<float> phi;
+    std::vector<uint64_t> dims;
+    if (!srt_io::read_scalar_field(argv[1], "/data", phi, dims, err)) {
+        std::cerr << err << std::endl;
+        return -1;
+    }
+    std::cout << "Data dimensions: " << dims[0] << " x " << dims[1] << " x " << dims[2] << std::endl;
+    std::cout << "Loading B field data from HDF5 file " << argv[2] << " ..." << std::endl;
+    std::vector<float> shift;
+    std::vector<uint64_t> bdims;
+    if (!srt_io::read_vector_field(argv[2], "/data", shift, bdims, err)) {
+        std::cerr << err << std::endl;
+        return -1;
+    }
+    if (bdims != dims) {
+        std::cerr << "Potential and B field dimensions differ" << std::endl;
+        return -1;
+    }
+
+    const int64_t npix = 12 * nside * nside;
+    int64_t istart = 0, count = npix;
+    if (batch >= 0) {
+        istart = batch * nside * nside;
+        count = nside * nside;
+        if (istart >= npix) {
+            std::cerr << "Batch exceeds total number of pixels" << std::endl;
+            return -1;
+        }
+    }
+
+    std::cout << "Allocating device memory and copying data to device..." << std::endl;
+    srt_context* ctx = nullptr;
+    int rc = srt_create((int)device, &ctx);
+    if (rc == SRT_OK)
+        rc = srt_upload(ctx, phi.data(), shift.data(), (int64_t)dims[2], (int64_t)dims[1], (int64_t)dims[0], 0,
+                        shift_s == "f16" ? SRT_SHIFT_F16_SCALED : SRT_SHIFT_F32);
+    if (rc != SRT_OK) {
+        std::cerr << "Device setup failed: " << srt_strerror(rc) << ": " << srt_last_error() << std::endl;
+        srt_destroy(ctx);
+        return -1;
+    }
+    std::vector<float>().swap(phi);
+    std::vector<float>().swap(shift);
+
+    std::cout << "Launching raytracing kernel (" << prec_s << ")..." << std::endl;
+    std::vector<float> maps((size_t)(6 * count));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int order = (batch < 0) ? SRT_ORDER_NESTED : SRT_ORDER_LINEAR;
+    rc = srt_trace_host(ctx, nside, istart, count, nullptr, (int)n_steps, nullptr, precision, order, maps.data(),
+                        nullptr);
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    float kernel_ms = 0.0f;
+    if (rc == SRT_OK) rc = srt_last_trace_ms(ctx, &kernel_ms);
+    srt_destroy(ctx);
+    if (rc != SRT_OK) {
+        std::cerr << "Raytracing failed: " << srt_strerror(rc) << ": " << srt_last_error() << std::endl;
+        return -1;
+    }
+    std::cout << "Raytracing completed: " << count << " rays x " << n_steps << " steps, kernel " << kernel_ms
+              << " ms (" << (double)count * (double)n_steps / (kernel_ms * 1e-3) << " ray-steps/s), "
+              << sec << " s with transfers." << std::endl;
+
+    std::cout << "Saving results to FITS files..." << std::endl;
+    const char* names[6] = {"dAmap", "ellipticity1map", "ellipticity2map",
+                            "rotationmap", "deflection1map", "deflection2map"};
+    const int64_t file_batch = batch < 0 ? 0 : batch;
+    std::vector<float> outmap((size_t)npix, 0.0f);
+    for (int m = 0; m < 6; ++m) {
+        std::memcpy(outmap.data() + istart, maps.data() + (size_t)m * count, (size_t)count * sizeof(float));
+        char fname[256];
+        std::snprintf(fname, sizeof(fname), "%s_batch%lld.fits", names[m], (long long)file_batch);
+        if (!srt_io::write_healpix_map(fname, outmap.data(), nside, false, "C", err)) {
+            std::cerr << err << std::endl;
+            return -1;
+        }
+    }
+    std::cout << "All done!" << std::endl;
+    return 0;
+}

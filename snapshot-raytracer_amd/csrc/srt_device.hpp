@@ -242,15 +242,17 @@ __device__ __forceinline__ Interp<R> tricubic(const float* __restrict__ base, si
 //       intermediates are rounded to; RA: right-hand-side assembly and the
 //       derivative type; kRefOffset: cell offset taken as float(x) - float(i)
 //       (.cu:228-230) instead of the exact x - i.
-template <class RI_, class RA_, bool kRefOffset_>
+//   kStateInLds: the kernel keeps the RK4 state and stage sum in LDS.
+template <class RI_, class RA_, bool kRefOffset_, bool kStateInLds_ = false>
 struct Form {
     using RI = RI_;
     using RA = RA_;
     static constexpr bool kRefOffset = kRefOffset_;
+    static constexpr bool kStateInLds = kStateInLds_;
 };
-using FormFP64 = Form<double, double, false>;
-using FormMixed = Form<float, float, false>;
-using FormRef = Form<float, double, true>;
+using FormFP64 = Form<double, double, false, true>;
+using FormMixed = Form<float, float, false, false>;
+using FormRef = Form<float, double, true, false>;
 
 template <class F>
 __device__ __forceinline__ typename F::RI cell_offset(double x, double t) {
@@ -317,6 +319,27 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
     }
     const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
 
+    // Catmull-Rom weights of the unstaggered offsets, shared by all four
+    // interpolations; each shift component adds the set of its own staggered axis
+    const Axis<RI> w0[3] = {cr_axis<RI>(t0[0]), cr_axis<RI>(t0[1]), cr_axis<RI>(t0[2])};
+
+    // potential at x (.cu:462)
+    RA phi, g[3], H[3][3];
+    {
+        const float* base = G.phi + (size_t)i0[0] + (size_t)i0[1] * py + (size_t)i0[2] * pz;
+        const Interp<RI> r = tricubic<RI>(base, py, pz, w0[0], w0[1], w0[2]);
+        phi = (RA)r.v;
+        g[0] = (RA)r.gx;
+        g[1] = (RA)r.gy;
+        g[2] = (RA)r.gz;
+        H[0][0] = (RA)r.hxx;
+        H[0][1] = H[1][0] = (RA)r.hxy;
+        H[0][2] = H[2][0] = (RA)r.hxz;
+        H[1][1] = (RA)r.hyy;
+        H[1][2] = H[2][1] = (RA)r.hyz;
+        H[2][2] = (RA)r.hzz;
+    }
+
     // shift: N_j = sum_a n_a dB_a/dx_j, E = 1/2 sum_a [e_a (n.gB_a) + n_a (e.gB_a)],
     // T_ij = sum_a n_a H^a_ij - 1/2 [(n H^i)_j + (n H^j)_i], L = sum_a n_a tr H^a
     RI N[3] = {RI(0), RI(0), RI(0)};
@@ -328,53 +351,32 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
         const int ix = (a == 0) ? i1[0] : i0[0];
         const int iy = (a == 1) ? i1[1] : i0[1];
         const int iz = (a == 2) ? i1[2] : i0[2];
-        const Axis<RI> ax = cr_axis<RI>(a == 0 ? t1[0] : t0[0]);
-        const Axis<RI> ay = cr_axis<RI>(a == 1 ? t1[1] : t0[1]);
-        const Axis<RI> az = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
+        const Axis<RI> ws = cr_axis<RI>(t1[a]);
         const float* base = G.b[a] + (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
-        const Interp<RI> r = tricubic<RI>(base, py, pz, ax, ay, az);
+        const Interp<RI> r = tricubic<RI>(base, py, pz, a == 0 ? ws : w0[0], a == 1 ? ws : w0[1],
+                                          a == 2 ? ws : w0[2]);
         Bv[a] = (RA)r.v;
-        const RA g[3] = {(RA)r.gx, (RA)r.gy, (RA)r.gz};
-        const RA H[3][3] = {{(RA)r.hxx, (RA)r.hxy, (RA)r.hxz},
-                            {(RA)r.hxy, (RA)r.hyy, (RA)r.hyz},
-                            {(RA)r.hxz, (RA)r.hyz, (RA)r.hzz}};
-        const RA ng = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
-        const RA e1g = e1[0] * g[0] + e1[1] * g[1] + e1[2] * g[2];
-        const RA e2g = e2[0] * g[0] + e2[1] * g[1] + e2[2] * g[2];
-        for (int j = 0; j < 3; ++j) N[j] = (RI)((RA)N[j] + n[a] * g[j]);
+        const RA gb[3] = {(RA)r.gx, (RA)r.gy, (RA)r.gz};
+        const RA Hb[3][3] = {{(RA)r.hxx, (RA)r.hxy, (RA)r.hxz},
+                             {(RA)r.hxy, (RA)r.hyy, (RA)r.hyz},
+                             {(RA)r.hxz, (RA)r.hyz, (RA)r.hzz}};
+        const RA ng = n[0] * gb[0] + n[1] * gb[1] + n[2] * gb[2];
+        const RA e1g = e1[0] * gb[0] + e1[1] * gb[1] + e1[2] * gb[2];
+        const RA e2g = e2[0] * gb[0] + e2[1] * gb[1] + e2[2] * gb[2];
+        for (int j = 0; j < 3; ++j) N[j] = (RI)((RA)N[j] + n[a] * gb[j]);
         E1 = (RI)((RA)E1 + RA(0.5) * (e1[a] * ng + n[a] * e1g));
         E2 = (RI)((RA)E2 + RA(0.5) * (e2[a] * ng + n[a] * e2g));
         RA nH[3];
-        for (int j = 0; j < 3; ++j) nH[j] = n[0] * H[0][j] + n[1] * H[1][j] + n[2] * H[2][j];
+        for (int j = 0; j < 3; ++j) nH[j] = n[0] * Hb[0][j] + n[1] * Hb[1][j] + n[2] * Hb[2][j];
         int k = 0;
         for (int i = 0; i < 3; ++i)
             for (int j = i; j < 3; ++j, ++k) {
-                RA c = n[a] * H[i][j];
+                RA c = n[a] * Hb[i][j];
                 if (i == a) c -= RA(0.5) * nH[j];
                 if (j == a) c -= RA(0.5) * nH[i];
                 T[k] = (RI)((RA)T[k] + c);
             }
-        L = (RI)((RA)L + n[a] * (H[0][0] + H[1][1] + H[2][2]));
-    }
-
-    // potential at x (.cu:462)
-    RA phi, g[3], H[3][3];
-    {
-        const Axis<RI> ax = cr_axis<RI>(t0[0]);
-        const Axis<RI> ay = cr_axis<RI>(t0[1]);
-        const Axis<RI> az = cr_axis<RI>(t0[2]);
-        const float* base = G.phi + (size_t)i0[0] + (size_t)i0[1] * py + (size_t)i0[2] * pz;
-        const Interp<RI> r = tricubic<RI>(base, py, pz, ax, ay, az);
-        phi = (RA)r.v;
-        g[0] = (RA)r.gx;
-        g[1] = (RA)r.gy;
-        g[2] = (RA)r.gz;
-        H[0][0] = (RA)r.hxx;
-        H[0][1] = H[1][0] = (RA)r.hxy;
-        H[0][2] = H[2][0] = (RA)r.hxz;
-        H[1][1] = (RA)r.hyy;
-        H[1][2] = H[2][1] = (RA)r.hyz;
-        H[2][2] = (RA)r.hzz;
+        L = (RI)((RA)L + n[a] * (Hb[0][0] + Hb[1][1] + Hb[2][2]));
     }
 
     // x' = -n (1 + 2 phi) - B, with the norm of n in FP64: the position decides

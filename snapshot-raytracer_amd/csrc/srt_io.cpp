@@ -1,0 +1,331 @@
+// HDF5 snapshot readers and the HEALPix FITS writer (host code).
+#include "srt_io.hpp"
+
+#include <hdf5.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+namespace srt_io {
+
+namespace {
+
+// Closes an HDF5 handle when it goes out of scope.
+struct H5Handle {
+    hid_t id = -1;
+    herr_t (*close)(hid_t) = nullptr;
+    H5Handle(hid_t i, herr_t (*c)(hid_t)) : id(i), close(c) {}
+    ~H5Handle() {
+        if (id >= 0 && close) close(id);
+    }
+    H5Handle(const H5Handle&) = delete;
+    H5Handle& operator=(const H5Handle&) = delete;
+};
+
+struct H5Silence {  // the readers report their own errors
+    H5E_auto2_t fn = nullptr;
+    void* data = nullptr;
+    H5Silence() {
+        H5Eget_auto2(H5E_DEFAULT, &fn, &data);
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    }
+    ~H5Silence() { H5Eset_auto2(H5E_DEFAULT, fn, data); }
+};
+
+bool open_dataset(const std::string& file, const std::string& dataset, H5Handle& f, H5Handle& d,
+                  std::string& err) {
+    f.id = H5Fopen(file.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+    f.close = H5Fclose;
+    if (f.id < 0) {
+        err = "Failed to open HDF5 file: " + file;
+        return false;
+    }
+    d.id = H5Dopen2(f.id, dataset.c_str(), H5P_DEFAULT);
+    d.close = H5Dclose;
+    if (d.id < 0) {
+        err = "Failed to open dataset: " + dataset;
+        return false;
+    }
+    return true;
+}
+
+bool rank3_dims(hid_t dset, std::vector<uint64_t>& dims, std::string& err) {
+    H5Handle space(H5Dget_space(dset), H5Sclose);
+    if (space.id < 0) {
+        err = "Failed to get dataspace";
+        return false;
+    }
+    const int rank = H5Sget_simple_extent_ndims(space.id);
+    if (rank != 3) {
+        err = "Expected 3D dataspace, got rank=" + std::to_string(rank);
+        return false;
+    }
+    hsize_t d[3];
+    if (H5Sget_simple_extent_dims(space.id, d, nullptr) < 0) {
+        err = "Failed to get dimensions";
+        return false;
+    }
+    dims.assign(d, d + 3);
+    return true;
+}
+
+}  // namespace
+
+bool read_scalar_field(const std::string& file, const std::string& dataset, std::vector<float>& data,
+                       std::vector<uint64_t>& dims, std::string& err) {
+    H5Silence quiet;
+    H5Handle f(-1, nullptr), d(-1, nullptr);
+    if (!open_dataset(file, dataset, f, d, err)) return false;
+    if (!rank3_dims(d.id, dims, err)) return false;
+    const uint64_t n = dims[0] * dims[1] * dims[2];
+    data.resize(n);
+    if (H5Dread(d.id, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, data.data()) < 0) {
+        err = "Failed to read data";
+        data.clear();
+        return false;
+    }
+    return true;
+}
+
+bool read_vector_field(const std::string& file, const std::string& dataset, std::vector<float>& data,
+                       std::vector<uint64_t>& dims, std::string& err) {
+    H5Silence quiet;
+    H5Handle f(-1, nullptr), d(-1, nullptr);
+    if (!open_dataset(file, dataset, f, d, err)) return false;
+    if (!rank3_dims(d.id, dims, err)) return false;
+    H5Handle type(H5Dget_type(d.id), H5Tclose);
+    if (type.id < 0) {
+        err = "Failed to get datatype";
+        return false;
+    }
+    if (H5Tget_class(type.id) != H5T_ARRAY) {
+        err = "Dataset is not an array type (expected array[3] of float).";
+        return false;
+    }
+    hsize_t adims[8] = {0};
+    const int andims = H5Tget_array_ndims(type.id);
+    if (andims < 1 || andims > 8 || H5Tget_array_dims2(type.id, adims) < 0) {
+        err = "Failed to get array dimensions.";
+        return false;
+    }
+    if (andims != 1 || adims[0] != 3) {
+        err = "Expected per-voxel array length 3, got length=" + std::to_string(adims[0]);
+        return false;
+    }
+    hsize_t three = 3;
+    H5Handle mem(H5Tarray_create2(H5T_NATIVE_FLOAT, 1, &three), H5Tclose);
+    if (mem.id < 0) {
+        err = "Failed to create memory array type";
+        return false;
+    }
+    data.resize(dims[0] * dims[1] * dims[2] * 3);
+    if (H5Dread(d.id, mem.id, H5S_ALL, H5S_ALL, H5P_DEFAULT, data.data()) < 0) {
+        err = "Failed to read vector field dataset.";
+        data.clear();
+        return false;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ FITS ----
+namespace {
+
+class FitsWriter {
+  public:
+    explicit FitsWriter(FILE* f) : f_(f) {}
+    // one 80-character header card
+    void card(const std::string& text) {
+        std::string c = text.substr(0, 80);
+        c.resize(80, ' ');
+        block_.append(c);
+    }
+    void key_str(const char* key, const std::string& value, const char* comment = "") {
+        std::string v = "'" + value;
+        while (v.size() < 9) v += ' ';  // fixed-format strings are at least 8 characters
+        v += "'";
+        card(fmt_key(key) + pad_value(v, false) + comment_of(comment));
+    }
+    void key_int(const char* key, int64_t value, const char* comment = "") {
+        card(fmt_key(key) + pad_value(std::to_string(value), true) + comment_of(comment));
+    }
+    void key_bool(const char* key, bool value, const char* comment = "") {
+        card(fmt_key(key) + pad_value(value ? "T" : "F", true) + comment_of(comment));
+    }
+    bool end_header() {
+        card("END");
+        while (block_.size() % 2880) block_.push_back(' ');
+        const bool ok = std::fwrite(block_.data(), 1, block_.size(), f_) == block_.size();
+        block_.clear();
+        return ok;
+    }
+
+  private:
+    static std::string fmt_key(const char* key) {
+        std::string k(key);
+        k.resize(8, ' ');
+        return k + "= ";
+    }
+    static std::string pad_value(const std::string& v, bool right) {
+        if (!right) return v.size() < 20 ? v + std::string(20 - v.size(), ' ') : v;
+        return v.size() < 20 ? std::string(20 - v.size(), ' ') + v : v;
+    }
+    static std::string comment_of(const char* c) { return (c && *c) ? std::string(" / ") + c : std::string(); }
+    FILE* f_;
+    std::string block_;
+};
+
+}  // namespace
+
+bool write_healpix_map(const std::string& file, const float* map, int64_t nside, bool nested,
+                       const std::string& coordsys, std::string& err) {
+    if (nside <= 0) {
+        err = "invalid nside";
+        return false;
+    }
+    const int64_t npix = 12 * nside * nside;
+    std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(file.c_str(), "wb"), std::fclose);
+    if (!f) {
+        err = "cannot create " + file;
+        return false;
+    }
+    FitsWriter w(f.get());
+    w.key_bool("SIMPLE", true, "conforms to FITS standard");
+    w.key_int("BITPIX", 8);
+    w.key_int("NAXIS", 0);
+    w.key_bool("EXTEND", true);
+    if (!w.end_header()) {
+        err = "write failed: " + file;
+        return false;
+    }
+    w.key_str("XTENSION", "BINTABLE", "binary table extension");
+    w.key_int("BITPIX", 8);
+    w.key_int("NAXIS", 2);
+    w.key_int("NAXIS1", 4, "bytes per row");
+    w.key_int("NAXIS2", npix, "rows = pixels");
+    w.key_int("PCOUNT", 0);
+    w.key_int("GCOUNT", 1);
+    w.key_int("TFIELDS", 1);
+    w.key_str("TTYPE1", "SIGNAL");
+    w.key_str("TFORM1", "1E", "float32");
+    w.key_str("EXTNAME", "xtension");
+    w.key_str("PIXTYPE", "HEALPIX", "HEALPIX Pixelisation");
+    w.key_str("ORDERING", nested ? "NESTED" : "RING", "Pixel ordering scheme");
+    w.key_int("NSIDE", nside, "Resolution parameter for HEALPIX");
+    w.key_int("FIRSTPIX", 0, "First pixel # (0 based)");
+    w.key_int("LASTPIX", npix - 1, "Last pixel # (0 based)");
+    w.key_str("INDXSCHM", "IMPLICIT", "Indexing: IMPLICIT or EXPLICIT");
+    w.key_str("OBJECT", "FULLSKY", "Sky coverage");
+    w.key_str("COORDSYS", coordsys, "Pixelisation coordinate system");
+    if (!w.end_header()) {
+        err = "write failed: " + file;
+        return false;
+    }
+    // big-endian IEEE float32 rows, padded to a 2880-byte block
+    std::vector<unsigned char> buf(1 << 20);
+    int64_t i = 0;
+    while (i < npix) {
+        const int64_t chunk = std::min<int64_t>(npix - i, (int64_t)buf.size() / 4);
+        for (int64_t k = 0; k < chunk; ++k) {
+            uint32_t u;
+            std::memcpy(&u, &map[i + k], 4);
+            buf[4 * k + 0] = (unsigned char)(u >> 24);
+            buf[4 * k + 1] = (unsigned char)(u >> 16);
+            buf[4 * k + 2] = (unsigned char)(u >> 8);
+            buf[4 * k + 3] = (unsigned char)u;
+        }
+        if (std::fwrite(buf.data(), 1, (size_t)(4 * chunk), f.get()) != (size_t)(4 * chunk)) {
+            err = "write failed: " + file;
+            return false;
+        }
+        i += chunk;
+    }
+    const int64_t bytes = 4 * npix;
+    const int64_t pad = (2880 - bytes % 2880) % 2880;
+    std::vector<unsigned char> zeros((size_t)pad, 0);
+    if (pad && std::fwrite(zeros.data(), 1, (size_t)pad, f.get()) != (size_t)pad) {
+        err = "write failed: " + file;
+        return false;
+    }
+    return true;
+}
+
+}  // namespace srt_io
+
+// ------------------------------------------------------------------ C ABI ----
+#include "../../include/srt_io.h"
+
+namespace {
+void copy_err(const std::string& e, char* err, size_t errlen) {
+    if (err && errlen) {
+        std::strncpy(err, e.c_str(), errlen - 1);
+        err[errlen - 1] = '\0';
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int srtio_read_scalar_field(const char* file, const char* dataset, float* out, uint64_t capacity, uint64_t* dims,
+                            char* err, size_t errlen) {
+    std::vector<float> data;
+    std::vector<uint64_t> d;
+    std::string e;
+    if (!file || !dataset || !dims) {
+        copy_err("null argument", err, errlen);
+        return 1;
+    }
+    if (!srt_io::read_scalar_field(file, dataset, data, d, e)) {
+        copy_err(e, err, errlen);
+        return 1;
+    }
+    for (int i = 0; i < 3; ++i) dims[i] = d[i];
+    if (out) {
+        if (capacity < data.size()) {
+            copy_err("output buffer too small", err, errlen);
+            return 1;
+        }
+        std::memcpy(out, data.data(), data.size() * sizeof(float));
+    }
+    return 0;
+}
+
+int srtio_read_vector_field(const char* file, const char* dataset, float* out, uint64_t capacity, uint64_t* dims,
+                            char* err, size_t errlen) {
+    std::vector<float> data;
+    std::vector<uint64_t> d;
+    std::string e;
+    if (!file || !dataset || !dims) {
+        copy_err("null argument", err, errlen);
+        return 1;
+    }
+    if (!srt_io::read_vector_field(file, dataset, data, d, e)) {
+        copy_err(e, err, errlen);
+        return 1;
+    }
+    for (int i = 0; i < 3; ++i) dims[i] = d[i];
+    if (out) {
+        if (capacity < data.size()) {
+            copy_err("output buffer too small", err, errlen);
+            return 1;
+        }
+        std::memcpy(out, data.data(), data.size() * sizeof(float));
+    }
+    return 0;
+}
+
+int srtio_write_healpix_map(const char* file, const float* map, int64_t nside, int nested, const char* coordsys,
+                            char* err, size_t errlen) {
+    std::string e;
+    if (!file || !map || !coordsys) {
+        copy_err("null argument", err, errlen);
+        return 1;
+    }
+    if (!srt_io::write_healpix_map(file, map, nside, nested != 0, coordsys, e)) {
+        copy_err(e, err, errlen);
+        return 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// Host I/O either side of the ray kernel: HDF5 snapshot readers and the
+// HEALPix FITS map writer.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace srt_io {
+
+// Scalar dataset (the Weyl potential): any numeric type, converted to float
+// by HDF5, rank 3 required.  dims in HDF5 order (slowest first).
+// Mirrors load_hdf5_array (snapshot-raytracer.cu:22-87); the reference does
+// not check the rank.  Returns false and fills err on failure.
+bool read_scalar_field(const std::string& file, const std::string& dataset, std::vector<float>& data,
+                       std::vector<uint64_t>& dims, std::string& err);
+
+// Vector dataset (the shift): rank-3 dataspace whose element type is a
+// one-dimensional H5T_ARRAY of length 3, read as float[3] per voxel,
+// interleaved.  Mirrors load_hdf5_vector_field (.cu:89-186).
+bool read_vector_field(const std::string& file, const std::string& dataset, std::vector<float>& data,
+                       std::vector<uint64_t>& dims, std::string& err);
+
+// Full-sky HEALPix map as a FITS binary table: primary HDU without data, then a
+// BINTABLE with one float32 column of 12 nside^2 rows and the HEALPix keywords
+// (PIXTYPE, ORDERING, NSIDE, FIRSTPIX, LASTPIX, INDXSCHM, OBJECT, COORDSYS).
+// Stands in for chealpix write_healpix_map(map, nside, name, nest, coordsys)
+// as called at .cu:947 (nest = 0, coordsys "C").  Overwrites the file.
+bool write_healpix_map(const std::string& file, const float* map, int64_t nside, bool nested,
+                       const std::string& coordsys, std::string& err);
+
+}  // namespace srt_io

@@ -132,15 +132,35 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
     double sp, cp;
     sincos(ph, &sp, &cp);
 
-    // state (.cu:560): x, n = -r, e1 = theta-hat, e2 = (sin phi, -cos phi, 0), D = 0, D' = -1
-    double y[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp, 0.0,
-                    0.0,      -1.0,     0.0,      0.0,     0.0,     0.0, 0.0};
+    // state (.cu:560): x, n = -r, e1 = theta-hat, e2 = (sin phi, -cos phi, 0), D = 0, D' = -1.
+    // With F::kStateInLds the state and the RK4 stage sum live in LDS (one
+    // column per lane, no sharing, no barriers) instead of registers: the
+    // right-hand side then has those registers for its interpolations.
+    extern __shared__ double lds_state[];
+    double y_reg[F::kStateInLds ? 1 : 19];
+    RA acc_reg[F::kStateInLds ? 1 : 19];
+    double* y_lds = lds_state + threadIdx.x;
+    RA* acc_lds = reinterpret_cast<RA*>(lds_state + 19 * kBlock) + threadIdx.x;
+    auto y = [&](int j) -> double& {
+        if constexpr (F::kStateInLds) return y_lds[j * kBlock];
+        else return y_reg[j];
+    };
+    auto acc = [&](int j) -> RA& {
+        if constexpr (F::kStateInLds) return acc_lds[j * kBlock];
+        else return acc_reg[j];
+    };
+    {
+        const double init[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp,
+                                 0.0,      0.0,      -1.0,     0.0,      0.0,     0.0, 0.0,    0.0};
+#pragma unroll
+        for (int j = 0; j < 19; ++j) y(j) = init[j];
+    }
     // RK4 (.cu:513-550).  Position and dA increments stay in FP64 in every
     // form: x' carries the transverse deflection and the FP64 dA decides the
     // dA > 1/2 branch of the right-hand side exactly as the FP64 oracle does.
     // Other derivatives and their stage sums are in RA.
     double accx[3], kx[3], accD, kD;
-    RA acc[19], k[19];
+    RA k[19];
     double comp[4] = {0.0, 0.0, 0.0, 0.0};  // compensated sums of x and dA (mixed form)
 
     // stage state y + c * k, then the right-hand side into (kxo, ko, kDo)
@@ -149,15 +169,15 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
         RA ys[10];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            xs[i] = y[i] + c * kxi[i];
-            ns[i] = y[3 + i] + c * (double)ki[3 + i];
+            xs[i] = y(i) + c * kxi[i];
+            ns[i] = y(3 + i) + c * (double)ki[3 + i];
         }
-        const double Ds = y[12] + c * kDi;
-        const double Dps = y[13] + c * (double)ki[13];
+        const double Ds = y(12) + c * kDi;
+        const double Dps = y(13) + c * (double)ki[13];
 #pragma unroll
-        for (int j = 0; j < 6; ++j) ys[j] = (RA)(y[6 + j] + c * (double)ki[6 + j]);
+        for (int j = 0; j < 6; ++j) ys[j] = (RA)(y(6 + j) + c * (double)ki[6 + j]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) ys[6 + j] = (RA)(y[14 + j] + c * (double)ki[14 + j]);
+        for (int j = 0; j < 4; ++j) ys[6 + j] = (RA)(y(14 + j) + c * (double)ki[14 + j]);
         srt::rhs<F>(A.G, xs, ns, Ds, Dps, ys, kxo, ko);
         kDo = -Dps;
     };
@@ -168,9 +188,11 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
             RA zk[19];
 #pragma unroll
             for (int j = 0; j < 19; ++j) zk[j] = RA(0);
-            eval(0.0, zx, zk, 0.0, accx, acc, accD);  // k1
+            eval(0.0, zx, zk, 0.0, accx, k, accD);  // k1
+#pragma unroll
+            for (int j = 0; j < 19; ++j) acc(j) = k[j];
         }
-        eval(0.5, accx, acc, accD, kx, k, kD);  // k2
+        eval(0.5, accx, k, accD, kx, k, kD);  // k2
         {
             const double kx2[3] = {kx[0], kx[1], kx[2]};
             RA k2[19];
@@ -180,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) accx[i] += 2.0 * kx[i];
 #pragma unroll
-            for (int j = 0; j < 19; ++j) acc[j] += RA(2) * k[j];
+            for (int j = 0; j < 19; ++j) acc(j) += RA(2) * k[j];
             accD += 2.0 * kD;
             eval(0.5, kx2, k2, kD2, kx, k, kD);  // k3
         }
@@ -193,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) accx[i] += 2.0 * kx[i];
 #pragma unroll
-            for (int j = 0; j < 19; ++j) acc[j] += RA(2) * k[j];
+            for (int j = 0; j < 19; ++j) acc(j) += RA(2) * k[j];
             accD += 2.0 * kD;
             eval(1.0, kx3, k3, kD3, kx, k, kD);  // k4
         }
@@ -203,37 +225,47 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
             for (int q = 0; q < 4; ++q) {
                 const int j = q < 3 ? q : 12;
                 const double inc = (q < 3 ? (accx[q] + kx[q]) : (accD + kD)) * sixth - comp[q];
-                const double t = y[j] + inc;
-                comp[q] = (t - y[j]) - inc;
-                y[j] = t;
+                const double yo = y(j);
+                const double t = yo + inc;
+                comp[q] = (t - yo) - inc;
+                y(j) = t;
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) y[i] += (accx[i] + kx[i]) * sixth;
-            y[12] += (accD + kD) * sixth;
+            for (int i = 0; i < 3; ++i) y(i) += (accx[i] + kx[i]) * sixth;
+            y(12) += (accD + kD) * sixth;
         }
 #pragma unroll
         for (int j = 3; j < 19; ++j)
-            if (j != 12) y[j] += (double)(acc[j] + k[j]) * sixth;
+            if (j != 12) y(j) += (double)(acc(j) + k[j]) * sixth;
     }
+    double yf[19];
+#pragma unroll
+    for (int j = 0; j < 19; ++j) yf[j] = y(j);
 
     // deflection: Delta = unit(x - observer) - r (.cu:566-575)
-    double d[3] = {y[0] - A.obs[0], y[1] - A.obs[1], y[2] - A.obs[2]};
+    double d[3] = {yf[0] - A.obs[0], yf[1] - A.obs[1], yf[2] - A.obs[2]};
     const double rn = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     d[0] = d[0] * rn - s * cp;
     d[1] = d[1] * rn - s * sp;
     d[2] = d[2] * rn - z;
     const int64_t C = A.count;
-    A.out[0 * C + slot] = (float)y[12];
-    A.out[1 * C + slot] = (float)y[16];
-    A.out[2 * C + slot] = (float)y[17];
-    A.out[3 * C + slot] = (float)y[18];
+    A.out[0 * C + slot] = (float)yf[12];
+    A.out[1 * C + slot] = (float)yf[16];
+    A.out[2 * C + slot] = (float)yf[17];
+    A.out[3 * C + slot] = (float)yf[18];
     A.out[4 * C + slot] = (float)(d[0] * z * cp + d[1] * z * sp - d[2] * s);
     A.out[5 * C + slot] = (float)(d[0] * sp - d[1] * cp);
     if (A.state) {
 #pragma unroll
-        for (int j = 0; j < 19; ++j) A.state[slot * 19 + j] = y[j];
+        for (int j = 0; j < 19; ++j) A.state[slot * 19 + j] = yf[j];
     }
+}
+
+// LDS bytes per block of trace_kernel<F>
+template <class F>
+constexpr unsigned lds_bytes() {
+    return F::kStateInLds ? (unsigned)(kBlock * 19 * (sizeof(double) + sizeof(typename F::RA))) : 0u;
 }
 
 }  // namespace
@@ -406,13 +438,16 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
         const int64_t blocks = (count + kBlock - 1) / kBlock;
         switch (precision) {
             case SRT_PREC_FP64:
-                hipLaunchKernelGGL(trace_kernel<srt::FormFP64>, dim3((unsigned)blocks), dim3(kBlock), 0, st, A);
+                hipLaunchKernelGGL(trace_kernel<srt::FormFP64>, dim3((unsigned)blocks), dim3(kBlock),
+                                   lds_bytes<srt::FormFP64>(), st, A);
                 break;
             case SRT_PREC_MIXED:
-                hipLaunchKernelGGL(trace_kernel<srt::FormMixed>, dim3((unsigned)blocks), dim3(kBlock), 0, st, A);
+                hipLaunchKernelGGL(trace_kernel<srt::FormMixed>, dim3((unsigned)blocks), dim3(kBlock),
+                                   lds_bytes<srt::FormMixed>(), st, A);
                 break;
             case SRT_PREC_REF:
-                hipLaunchKernelGGL(trace_kernel<srt::FormRef>, dim3((unsigned)blocks), dim3(kBlock), 0, st, A);
+                hipLaunchKernelGGL(trace_kernel<srt::FormRef>, dim3((unsigned)blocks), dim3(kBlock),
+                                   lds_bytes<srt::FormRef>(), st, A);
                 break;
             default:
                 return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");

@@ -1,0 +1,75 @@
+"""The command-line drop-in end to end on the GPU: HDF5 snapshot in, six RING
+FITS maps out (snapshot-raytracer.cu:600-1006, README.md:27-49).  Maps must
+equal the library's own trace of the same pixels bit for bit, be exactly zero
+outside the batch, and match the FP64 oracle within the kernel's tolerance."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "snapshot-raytracer_amd", "bin", "snapshot-raytracer")
+NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "deflection1map", "deflection2map")
+
+
+@pytest.fixture(scope="module")
+def files(tmp_path_factory, fields):
+    from snapshot_raytracer_amd import snapshot_io
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built (no HDF5)")
+    d = tmp_path_factory.mktemp("cli")
+    phi, shift = fields.snapshot(32, 21)
+    snapshot_io.write_potential(str(d / "phi.h5"), phi.numpy())
+    snapshot_io.write_shift(str(d / "B.h5"), fields.interleave_shift(shift).numpy())
+    return d, phi.numpy(), shift.numpy(), fields.interleave_shift(shift).numpy()
+
+
+def run(d, args, env=None):
+    print(f"[gpu] cli {args} {env or ''}", flush=True)
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([CLI, "phi.h5", "B.h5"] + args, cwd=d, capture_output=True, text=True, timeout=120,
+                          env=e)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_cli_full_sky_matches_library(srt, files, precision):
+    from snapshot_raytracer_amd import snapshot_io
+    d, phi, shift, shift_il = files
+    nside, steps = 4, 10
+    r = run(d, [str(nside), str(steps)], {"SRT_PRECISION": precision})
+    assert r.returncode == 0, r.stderr
+    assert "All done!" in r.stdout
+    tr = srt.Tracer(0)
+    tr.upload(phi, shift_il)
+    ref = tr.trace(nside, steps, precision=srt.PRECISIONS[precision])
+    tr.close()
+    for m, name in enumerate(NAMES):
+        got, hdr = snapshot_io.read_healpix_fits(str(d / f"{name}_batch0.fits"))
+        assert hdr["NSIDE"] == nside and hdr["ORDERING"] == "RING" and hdr["COORDSYS"] == "C"
+        assert np.array_equal(got, ref[m]), name
+
+
+def test_cli_batch_masks_and_oracle(srt, oracle, files):
+    from snapshot_raytracer_amd import snapshot_io
+    d, phi, shift, shift_il = files
+    nside, steps, batch = 8, 12, 7
+    r = run(d, [str(nside), str(steps), str(batch)], {"SRT_PRECISION": "fp64"})
+    assert r.returncode == 0, r.stderr
+    lo, cnt = srt.batch_range(nside, batch)
+    ref, _ = oracle.trace(oracle.Fields(phi, shift), nside, np.arange(lo, lo + cnt), steps)
+    for m, name in enumerate(NAMES):
+        got, _ = snapshot_io.read_healpix_fits(str(d / f"{name}_batch{batch}.fits"))
+        assert np.all(got[:lo] == 0) and np.all(got[lo + cnt:] == 0)
+        np.testing.assert_allclose(got[lo:lo + cnt], ref[m], rtol=3e-7, atol=1e-30)
+
+
+def test_cli_reference_storage_mode(files):
+    d = files[0]
+    r = run(d, ["4", "6"], {"SRT_PRECISION": "ref", "SRT_SHIFT": "f16"})
+    assert r.returncode == 0, r.stderr
+    r = run(d, ["4", "6"], {"SRT_PRECISION": "bogus"})
+    assert r.returncode == 255
