@@ -121,6 +121,25 @@ int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t c
                    int n_steps, const double* observer, int precision, int ray_order, float* out,
                    double* final_state);
 
+/* Adjoint of the FP64 trace (no counterpart in the reference; the map-loss
+ * gradient extension of raytracing_kernel, .cu:552-583).  For the rays of
+ * srt_trace (same pix_begin/count/pix_list/observer meaning), runs the FP64
+ * integrator, storing the state every checkpoint_every steps, and returns:
+ *   maps64 (optional): the six maps in FP64, map-major 6 * count;
+ *   if map_grad (6 * count doubles, dL/d(map), map-major) is not null:
+ *     grad_phi       (nx*ny*nz doubles)   dL/d(potential grid value),
+ *     grad_shift_xyz (nx*ny*nz*3 doubles) dL/d(shift grid value), interleaved
+ *   by a reverse sweep that recomputes each checkpoint segment's states,
+ *   propagates dL/dy through RK4 steps, right-hand sides and tricubic
+ *   interpolations, and adds the tap gradients with FP64 atomics.
+ * All pointers are device pointers; asynchronous on `stream`.  Work buffers
+ * (checkpoints, segment states, padded gradient grids) are owned by the
+ * context and reused.  The gradient is with respect to the grid values the
+ * context holds (the binary16-quantised ones for SRT_SHIFT_F16_SCALED). */
+int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+                      int n_steps, const double* observer, int checkpoint_every, const double* map_grad,
+                      double* maps64, double* grad_phi, double* grad_shift_xyz, void* stream);
+
 /* Milliseconds between HIP events recorded on the launch stream just before
  * and just after the kernel launches of the last srt_trace call of this
  * context (blocks until that work is done). */

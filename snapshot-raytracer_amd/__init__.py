@@ -36,7 +36,7 @@ MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "defl
 
 # C-ABI functions declared in include/srt.h
 EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_trace",
-           "srt_trace_host", "srt_last_trace_ms", "srt_grid_info", "srt_version")
+           "srt_trace_host", "srt_trace_adjoint", "srt_last_trace_ms", "srt_grid_info", "srt_version")
 
 _lib = None
 
@@ -67,6 +67,7 @@ def lib():
         L.srt_upload.argtypes = [P, P, P, i64, i64, i64, i32, i32]
         L.srt_trace.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P, P]
         L.srt_trace_host.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P]
+        L.srt_trace_adjoint.argtypes = [P, i64, i64, i64, P, i32, P, i32, P, P, P, P, P]
         L.srt_last_trace_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.srt_grid_info.argtypes = [P, P, P, P, P]
         _lib = L
@@ -162,6 +163,33 @@ class Tracer:
         _check(lib().srt_trace(self._ctx, nside, pix_begin, count, _ptr(pix_list), n_steps, _ptr(obs),
                                precision, order, _ptr(out), _ptr(final_state), st), "srt_trace")
 
+    def trace_adjoint(self, nside: int, n_steps: int, map_grad=None, *, checkpoint_every: int = 16,
+                      pix_begin: int = 0, count: int | None = None, pix_list=None, observer=None, stream=None):
+        """FP64 maps and, if map_grad is given, dL/d(grids) by the adjoint kernel.
+
+        map_grad: float64 CUDA tensor (6, count) = dL/d(maps).  Returns
+        (maps64 (6, count), grad_phi (n, n, n) or None, grad_shift (n, n, n, 3) or None),
+        CUDA float64 tensors.
+        """
+        import torch
+        if count is None:
+            count = pix_list.numel() if pix_list is not None else 12 * nside * nside - pix_begin
+        dev = torch.device("cuda", self.device)
+        maps64 = torch.empty((6, count), dtype=torch.float64, device=dev)
+        gphi = gshift = None
+        if map_grad is not None:
+            map_grad = map_grad.to(device=dev, dtype=torch.float64).contiguous()
+            if tuple(map_grad.shape) != (6, count):
+                raise ValueError("map_grad must be (6, count)")
+            gphi = torch.empty((self.n,) * 3, dtype=torch.float64, device=dev)
+            gshift = torch.empty((self.n,) * 3 + (3,), dtype=torch.float64, device=dev)
+        obs = None if observer is None else np.ascontiguousarray(observer, dtype=np.float64)
+        st = ctypes.c_void_p((stream or torch.cuda.current_stream(dev)).cuda_stream)
+        _check(lib().srt_trace_adjoint(self._ctx, nside, pix_begin, count, _ptr(pix_list), n_steps, _ptr(obs),
+                                       checkpoint_every, _ptr(map_grad), _ptr(maps64), _ptr(gphi), _ptr(gshift),
+                                       st), "srt_trace_adjoint")
+        return maps64, gphi, gshift
+
     def last_trace_ms(self) -> float:
         ms = ctypes.c_float()
         _check(lib().srt_last_trace_ms(self._ctx, ctypes.byref(ms)), "srt_last_trace_ms")
@@ -199,3 +227,34 @@ def full_sky_maps(nside: int, batch: int | None, maps: np.ndarray) -> np.ndarray
 
 def version() -> str:
     return lib().srt_version().decode()
+
+
+def ray_maps(phi, shift_xyz, nside: int, n_steps: int, *, tracer: Tracer | None = None, checkpoint_every: int = 16,
+             pix_begin: int = 0, count: int | None = None, pix_list=None, observer=None):
+    """Differentiable FP64 maps (6, count) of a snapshot, for torch autograd.
+
+    phi (n, n, n) and shift_xyz (n, n, n, 3) are CUDA tensors (stored on the
+    device as float32, the precision the grids have in the ray tracer); the
+    backward pass runs the adjoint kernel (srt_trace_adjoint).
+    """
+    import torch
+
+    class _RayMaps(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, phi_t, shift_t):
+            tr = tracer or Tracer(phi_t.device.index or 0)
+            tr.upload(phi_t.detach().float().contiguous(), shift_t.detach().float().contiguous())
+            maps, _, _ = tr.trace_adjoint(nside, n_steps, None, checkpoint_every=checkpoint_every,
+                                          pix_begin=pix_begin, count=count, pix_list=pix_list, observer=observer)
+            ctx.tr = tr
+            ctx.dtypes = (phi_t.dtype, shift_t.dtype)
+            return maps
+
+        @staticmethod
+        def backward(ctx, grad_maps):
+            _, gphi, gshift = ctx.tr.trace_adjoint(nside, n_steps, grad_maps, checkpoint_every=checkpoint_every,
+                                                   pix_begin=pix_begin, count=count, pix_list=pix_list,
+                                                   observer=observer)
+            return gphi.to(ctx.dtypes[0]), gshift.to(ctx.dtypes[1])
+
+    return _RayMaps.apply(phi, shift_xyz)

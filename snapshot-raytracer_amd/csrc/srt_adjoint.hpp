@@ -1,0 +1,543 @@
+// Reverse-mode (adjoint) building blocks of the FP64 ray integrator.
+//
+// For a loss L(maps) the adjoint kernel propagates dL/dy backwards through the
+// RK4 steps (.cu:513-550), the right-hand side (.cu:315-510) and the tricubic
+// interpolation (.cu:215-312), and scatters dL/d(grid value) with atomics.
+// Everything here is FP64.  The right-hand side is restated with its
+// intermediates kept (rhs_forward) so its vector-Jacobian product (rhs_vjp)
+// can be written out step by step; the interpolation adjoint has two parts:
+// the transpose of the tap contraction (gradient with respect to the 64 tap
+// values) and the derivative with respect to the position (third derivatives
+// of the Catmull-Rom interpolant contracted with the Hessian adjoint).
+#pragma once
+
+#include "srt_device.hpp"
+
+namespace srt {
+namespace adj {
+
+// Catmull-Rom weights of orders 0..3 along one axis (third-derivative weights
+// are constant: d3/dt3 of the value weights).
+struct Axis4 {
+    double w[4][4];  // w[order][tap]
+};
+
+__device__ __forceinline__ Axis4 cr_axis4(double t) {
+    const Axis<double> a = cr_axis<double>(t);
+    Axis4 r;
+    for (int i = 0; i < 4; ++i) {
+        r.w[0][i] = a.w[i];
+        r.w[1][i] = a.d[i];
+        r.w[2][i] = a.dd[i];
+    }
+    r.w[3][0] = -3.0;
+    r.w[3][1] = 9.0;
+    r.w[3][2] = -9.0;
+    r.w[3][3] = 3.0;
+    return r;
+}
+
+// Interpolated value, gradient, Hessian (xx xy xz yy yz zz) and third
+// derivatives (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz).
+struct Interp3 {
+    double v, g[3], h[6], t[10];
+};
+
+// derivative orders (x, y, z) of the 20 outputs: value, gradient, Hessian, third derivatives
+__constant__ static const signed char kOrders[20][3] = {
+    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2},
+    {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
+
+__device__ __forceinline__ Interp3 tricubic3(const float* __restrict__ base, size_t py, size_t pz, const Axis4& ax,
+                                             const Axis4& ay, const Axis4& az) {
+    double o[20];
+    for (int k = 0; k < 20; ++k) o[k] = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        // y-pass results for this z slice, indexed [ox][oy]
+        double Y[4][4];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) Y[i][j] = 0.0;
+        for (int b = 0; b < 4; ++b) {
+            const Row4 q = load_row(base + b * py + c * pz);
+            const double p[4] = {q.x, q.y, q.z, q.w};
+            double X[4];
+            for (int ox = 0; ox < 4; ++ox)
+                X[ox] = ax.w[ox][0] * p[0] + ax.w[ox][1] * p[1] + ax.w[ox][2] * p[2] + ax.w[ox][3] * p[3];
+            for (int ox = 0; ox < 4; ++ox)
+                for (int oy = 0; ox + oy < 4; ++oy) Y[ox][oy] += ay.w[oy][b] * X[ox];
+        }
+        for (int k = 0; k < 20; ++k) {
+            const int ox = kOrders[k][0], oy = kOrders[k][1], oz = kOrders[k][2];
+            o[k] += az.w[oz][c] * Y[ox][oy];
+        }
+    }
+    Interp3 r;
+    r.v = o[0];
+    for (int i = 0; i < 3; ++i) r.g[i] = o[1 + i];
+    for (int i = 0; i < 6; ++i) r.h[i] = o[4 + i];
+    for (int i = 0; i < 10; ++i) r.t[i] = o[10 + i];
+    return r;
+}
+
+// index of the third derivative d3/(dx_i dx_j dx_k) in Interp3::t
+__device__ __forceinline__ int t3idx(int i, int j, int k) {
+    // sort i <= j <= k
+    if (i > j) { const int s = i; i = j; j = s; }
+    if (j > k) { const int s = j; j = k; k = s; }
+    if (i > j) { const int s = i; i = j; j = s; }
+    // xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+    static const signed char tab[3][3][3] = {
+        {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}, {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}}, {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
+    return tab[i][j][k];
+}
+
+__device__ __forceinline__ int h6(int i, int j) {
+    return (i == j) ? (i == 0 ? 0 : (i == 1 ? 3 : 5)) : ((i + j == 1) ? 1 : ((i + j == 2) ? 2 : 4));
+}
+
+// Adds the gradient of  sum_o bar[o] * interp_o  with respect to the 64 tap
+// values (o = value, gradient, Hessian) into the padded gradient grid, with
+// one FP64 atomic per tap.  When every lane of the wavefront sits in the same
+// stencil (the rays near the observer), the 64 contributions are first summed
+// across the wavefront and one lane issues the atomics.
+__device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, size_t off, size_t py, size_t pz,
+                                             const Axis4& ax, const Axis4& ay, const Axis4& az, const double bar[10]) {
+    const bool uniform = __builtin_amdgcn_readfirstlane((int)off) == (int)off &&
+                         __builtin_amdgcn_readfirstlane((int)(off >> 32)) == (int)(off >> 32);
+    const bool all_uniform = __all(uniform);
+    // z-pass transpose: per slice c, adjoints of the six y-pass sums
+    for (int c = 0; c < 4; ++c) {
+        const double yv = az.w[0][c] * bar[0] + az.w[1][c] * bar[3] + az.w[2][c] * bar[9];
+        const double yd = az.w[0][c] * bar[2] + az.w[1][c] * bar[8];
+        const double ydd = az.w[0][c] * bar[7];
+        const double xd = az.w[0][c] * bar[1] + az.w[1][c] * bar[6];
+        const double xdyd = az.w[0][c] * bar[5];
+        const double xdd = az.w[0][c] * bar[4];
+        for (int b = 0; b < 4; ++b) {
+            const double rv = ay.w[0][b] * yv + ay.w[1][b] * yd + ay.w[2][b] * ydd;
+            const double rd = ay.w[0][b] * xd + ay.w[1][b] * xdyd;
+            const double rdd = ay.w[0][b] * xdd;
+            for (int a = 0; a < 4; ++a) {
+                double pb = ax.w[0][a] * rv + ax.w[1][a] * rd + ax.w[2][a] * rdd;
+                double* dst = gbase_dev + off + a + b * py + c * pz;
+                if (all_uniform) {
+                    for (int m = 32; m >= 1; m >>= 1) pb += __shfl_xor(pb, m);
+                    if ((threadIdx.x & 63) == 0) atomicAdd(dst, pb);
+                } else {
+                    atomicAdd(dst, pb);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ RHS forward ----
+struct RhsTape {
+    // stage inputs
+    double nd[3], e1d[3], e2d[3], D, Dp, s1, s2, eps1, eps2;
+    // normalised basis
+    double rn, r1, r2, n[3], e1[3], e2[3];
+    // interpolations: potential and the three shift components
+    double phi, g[3], H[3][3];
+    double Bv[3], gb[3][3], Hb[3][3][3];
+    // intermediates
+    double N[3], E1, E2, T[3][3], L, gam, lapse, v[3], nv, c1, c2, PH, Hm[3][3], S1H, S2H, q;
+    bool br;
+};
+
+struct Stencil {
+    size_t off;       // offset of tap (-1,-1,-1) in the padded grid
+    Axis4 ax, ay, az;
+};
+
+__device__ __forceinline__ void locate_stencils(const Grids& G, const double x[3], Stencil st[4]) {
+    int i0[3], i1[3];
+    double t0[3], t1[3];
+    for (int a = 0; a < 3; ++a) {
+        double t;
+        i0[a] = locate(x[a], G.n, G.inv_n, t);
+        t0[a] = t;
+        const bool upper = t >= 0.5;
+        const int i = upper ? i0[a] : i0[a] - 1;
+        i1[a] = i < 0 ? i + G.n : i;
+        t1[a] = upper ? t - 0.5 : t + 0.5;
+    }
+    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
+    // st[0] = potential; st[1 + a] = shift component a (staggered along a)
+    for (int f = 0; f < 4; ++f) {
+        const int a = f - 1;
+        const int ix = (a == 0) ? i1[0] : i0[0];
+        const int iy = (a == 1) ? i1[1] : i0[1];
+        const int iz = (a == 2) ? i1[2] : i0[2];
+        st[f].off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
+        st[f].ax = cr_axis4(a == 0 ? t1[0] : t0[0]);
+        st[f].ay = cr_axis4(a == 1 ? t1[1] : t0[1]);
+        st[f].az = cr_axis4(a == 2 ? t1[2] : t0[2]);
+    }
+}
+
+// Right-hand side at stage state s (19, FP64), keeping the intermediates and
+// the third derivatives of the interpolations (t3[field][10]).
+__device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, const Stencil st[4], RhsTape& T,
+                                            double t3[4][10], double dy[19]) {
+    for (int i = 0; i < 3; ++i) {
+        T.nd[i] = s[3 + i];
+        T.e1d[i] = s[6 + i];
+        T.e2d[i] = s[9 + i];
+    }
+    T.D = s[12];
+    T.Dp = s[13];
+    T.s1 = s[14];
+    T.s2 = s[15];
+    T.eps1 = s[16];
+    T.eps2 = s[17];
+    T.rn = 1.0 / sqrt(T.nd[0] * T.nd[0] + T.nd[1] * T.nd[1] + T.nd[2] * T.nd[2]);
+    T.r1 = 1.0 / sqrt(T.e1d[0] * T.e1d[0] + T.e1d[1] * T.e1d[1] + T.e1d[2] * T.e1d[2]);
+    T.r2 = 1.0 / sqrt(T.e2d[0] * T.e2d[0] + T.e2d[1] * T.e2d[1] + T.e2d[2] * T.e2d[2]);
+    for (int i = 0; i < 3; ++i) {
+        T.n[i] = T.nd[i] * T.rn;
+        T.e1[i] = T.e1d[i] * T.r1;
+        T.e2[i] = T.e2d[i] * T.r2;
+    }
+    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
+    const float* grids[4] = {G.phi, G.b[0], G.b[1], G.b[2]};
+    for (int f = 0; f < 4; ++f) {
+        const Interp3 r = tricubic3(grids[f] + st[f].off, py, pz, st[f].ax, st[f].ay, st[f].az);
+        for (int k = 0; k < 10; ++k) t3[f][k] = r.t[k];
+        double H[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) H[i][j] = r.h[h6(i, j)];
+        if (f == 0) {
+            T.phi = r.v;
+            for (int i = 0; i < 3; ++i) {
+                T.g[i] = r.g[i];
+                for (int j = 0; j < 3; ++j) T.H[i][j] = H[i][j];
+            }
+        } else {
+            T.Bv[f - 1] = r.v;
+            for (int i = 0; i < 3; ++i) {
+                T.gb[f - 1][i] = r.g[i];
+                for (int j = 0; j < 3; ++j) T.Hb[f - 1][i][j] = H[i][j];
+            }
+        }
+    }
+    const double* n = T.n;
+    const double* e1 = T.e1;
+    const double* e2 = T.e2;
+    T.E1 = 0.0;
+    T.E2 = 0.0;
+    T.L = 0.0;
+    for (int j = 0; j < 3; ++j) T.N[j] = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T.T[i][j] = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double* gb = T.gb[a];
+        double ng = 0, e1g = 0, e2g = 0;
+        for (int j = 0; j < 3; ++j) {
+            ng += n[j] * gb[j];
+            e1g += e1[j] * gb[j];
+            e2g += e2[j] * gb[j];
+            T.N[j] += n[a] * gb[j];
+        }
+        T.E1 += 0.5 * (e1[a] * ng + n[a] * e1g);
+        T.E2 += 0.5 * (e2[a] * ng + n[a] * e2g);
+        T.L += n[a] * (T.Hb[a][0][0] + T.Hb[a][1][1] + T.Hb[a][2][2]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) T.T[i][j] += n[a] * T.Hb[a][i][j];
+    }
+    // - 1/2 [(n H^i)_j + (n H^j)_i]
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double nhij = 0, nhji = 0;
+            for (int k = 0; k < 3; ++k) {
+                nhij += n[k] * T.Hb[i][k][j];
+                nhji += n[k] * T.Hb[j][k][i];
+            }
+            T.T[i][j] -= 0.5 * (nhij + nhji);
+        }
+    T.gam = 0.0;
+    for (int i = 0; i < 3; ++i) T.gam += n[i] * T.g[i] + 0.25 * n[i] * T.N[i];
+    T.lapse = 1.0 + 2.0 * T.phi;
+    for (int i = 0; i < 3; ++i) dy[i] = -n[i] * T.lapse - T.Bv[i];
+    T.nv = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        T.v[i] = 2.0 * T.g[i] + T.N[i];
+        T.nv += n[i] * T.v[i];
+    }
+    for (int i = 0; i < 3; ++i) dy[3 + i] = T.v[i] - n[i] * T.nv;
+    T.c1 = T.E1;
+    T.c2 = T.E2;
+    for (int i = 0; i < 3; ++i) {
+        T.c1 += 2.0 * e1[i] * T.g[i];
+        T.c2 += 2.0 * e2[i] * T.g[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        dy[6 + i] = -n[i] * T.c1;
+        dy[9 + i] = -n[i] * T.c2;
+    }
+    dy[12] = -T.Dp;
+    T.PH = -(T.H[0][0] + T.H[1][1] + T.H[2][2]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T.PH += n[i] * T.H[i][j] * n[j];
+    dy[13] = -4.0 * T.Dp * T.gam - T.PH * T.D + 0.5 * T.L * T.D;
+    T.S1H = 0.0;
+    T.S2H = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            T.Hm[i][j] = T.H[i][j] + 0.5 * T.T[i][j];
+            T.S1H += (e1[i] * e1[j] - e2[i] * e2[j]) * T.Hm[i][j];
+            T.S2H += (e1[i] * e2[j] + e2[i] * e1[j]) * T.Hm[i][j];
+        }
+    const double D2 = T.D * T.D;
+    dy[14] = 4.0 * T.s1 * T.gam + T.S1H * D2;
+    dy[15] = 4.0 * T.s2 * T.gam + T.S2H * D2;
+    T.br = T.D > 0.5;
+    T.q = sqrt(4.0 + T.eps1 * T.eps1 + T.eps2 * T.eps2);
+    if (T.br) {
+        dy[13] += (T.s1 * T.s1 + T.s2 * T.s2) / (D2 * T.D);
+        dy[16] = -2.0 * T.s1 * T.q / D2;
+        dy[17] = -2.0 * T.s2 * T.q / D2;
+        dy[18] = -(T.s1 * T.eps2 - T.s2 * T.eps1) / (D2 * (2.0 + T.q));
+    } else {
+        dy[16] = dy[17] = dy[18] = 0.0;
+    }
+}
+
+// Vector-Jacobian product of the right-hand side: given a = dL/d(dy) (19),
+// returns sbar = dL/ds (19) and scatters dL/d(grid) for the four fields.
+__device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], const RhsTape& T,
+                                        const double t3[4][10], const double a[19], double* const gdst[4],
+                                        double sbar[19]) {
+    const double* n = T.n;
+    const double* e1 = T.e1;
+    const double* e2 = T.e2;
+    double nb[3] = {0, 0, 0}, e1b[3] = {0, 0, 0}, e2b[3] = {0, 0, 0};
+    double Db = 0, Dpb = 0, s1b = 0, s2b = 0, eps1b = 0, eps2b = 0;
+    double phib = 0, gb_[3] = {0, 0, 0}, Hbar[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double Bvb[3] = {0, 0, 0}, gbb[3][3] = {{0}}, Hbb[3][3][3] = {{{0}}};
+    double Nb[3] = {0, 0, 0}, E1b = 0, E2b = 0, Tb[3][3] = {{0}}, Lb = 0, gamb = 0;
+    const double D = T.D, D2 = D * D;
+
+    // omega', eps', and the sigma^2 / D^3 term (branch D > 1/2)
+    if (T.br) {
+        double qb = 0;
+        s1b += a[16] * (-2.0 * T.q / D2);
+        qb += a[16] * (-2.0 * T.s1 / D2);
+        Db += a[16] * (4.0 * T.s1 * T.q / (D2 * D));
+        s2b += a[17] * (-2.0 * T.q / D2);
+        qb += a[17] * (-2.0 * T.s2 / D2);
+        Db += a[17] * (4.0 * T.s2 * T.q / (D2 * D));
+        const double num = T.s1 * T.eps2 - T.s2 * T.eps1;
+        const double den = D2 * (2.0 + T.q);
+        const double numb = -a[18] / den;
+        const double denb = a[18] * num / (den * den);
+        s1b += numb * T.eps2;
+        eps2b += numb * T.s1;
+        s2b -= numb * T.eps1;
+        eps1b -= numb * T.s2;
+        Db += denb * 2.0 * D * (2.0 + T.q);
+        qb += denb * D2;
+        const double ss = T.s1 * T.s1 + T.s2 * T.s2;
+        s1b += a[13] * 2.0 * T.s1 / (D2 * D);
+        s2b += a[13] * 2.0 * T.s2 / (D2 * D);
+        Db += a[13] * (-3.0 * ss / (D2 * D2));
+        eps1b += qb * T.eps1 / T.q;
+        eps2b += qb * T.eps2 / T.q;
+    }
+    // sigma' = 4 sigma Gamma + S:Hm D^2
+    s1b += a[14] * 4.0 * T.gam;
+    s2b += a[15] * 4.0 * T.gam;
+    gamb += a[14] * 4.0 * T.s1 + a[15] * 4.0 * T.s2;
+    const double S1Hb = a[14] * D2, S2Hb = a[15] * D2;
+    Db += a[14] * 2.0 * T.S1H * D + a[15] * 2.0 * T.S2H * D;
+    double Hmb[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Hmb[i][j] = S1Hb * (e1[i] * e1[j] - e2[i] * e2[j]) + S2Hb * (e1[i] * e2[j] + e2[i] * e1[j]);
+    for (int i = 0; i < 3; ++i) {
+        double hm1 = 0, hm2 = 0;
+        for (int j = 0; j < 3; ++j) {
+            hm1 += T.Hm[i][j] * e1[j];
+            hm2 += T.Hm[i][j] * e2[j];
+        }
+        e1b[i] += S1Hb * 2.0 * hm1 + S2Hb * 2.0 * hm2;
+        e2b[i] += -S1Hb * 2.0 * hm2 + S2Hb * 2.0 * hm1;
+    }
+    // Hm = H + T/2
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            Hbar[i][j] += Hmb[i][j];
+            Tb[i][j] += 0.5 * Hmb[i][j];
+        }
+    // D'' = -4 D' Gamma - PH D + L D / 2
+    Dpb += a[13] * (-4.0 * T.gam);
+    gamb += a[13] * (-4.0 * T.Dp);
+    const double PHb = a[13] * (-D);
+    Db += a[13] * (-T.PH + 0.5 * T.L);
+    Lb += a[13] * 0.5 * D;
+    // PH = n^T H n - tr H
+    for (int i = 0; i < 3; ++i) {
+        double hn = 0;
+        for (int j = 0; j < 3; ++j) {
+            Hbar[i][j] += PHb * (n[i] * n[j] - (i == j ? 1.0 : 0.0));
+            hn += T.H[i][j] * n[j];
+        }
+        nb[i] += PHb * 2.0 * hn;
+    }
+    // D' component
+    Dpb -= a[12];
+    // e' = -n c
+    double c1b = 0, c2b = 0;
+    for (int i = 0; i < 3; ++i) {
+        nb[i] -= a[6 + i] * T.c1 + a[9 + i] * T.c2;
+        c1b -= a[6 + i] * n[i];
+        c2b -= a[9 + i] * n[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        e1b[i] += c1b * 2.0 * T.g[i];
+        e2b[i] += c2b * 2.0 * T.g[i];
+        gb_[i] += c1b * 2.0 * e1[i] + c2b * 2.0 * e2[i];
+    }
+    E1b += c1b;
+    E2b += c2b;
+    // n' = v - n (n . v), v = 2 g + N
+    double vb[3], nvb = 0;
+    for (int i = 0; i < 3; ++i) {
+        vb[i] = a[3 + i];
+        nb[i] -= a[3 + i] * T.nv;
+        nvb -= a[3 + i] * n[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        nb[i] += nvb * T.v[i];
+        vb[i] += nvb * n[i];
+        gb_[i] += 2.0 * vb[i];
+        Nb[i] += vb[i];
+    }
+    // x' = -n lapse - B
+    double lapseb = 0;
+    for (int i = 0; i < 3; ++i) {
+        nb[i] -= a[i] * T.lapse;
+        lapseb -= a[i] * n[i];
+        Bvb[i] -= a[i];
+    }
+    phib += 2.0 * lapseb;
+    // Gamma = n.g + n.N / 4
+    for (int i = 0; i < 3; ++i) {
+        nb[i] += gamb * (T.g[i] + 0.25 * T.N[i]);
+        gb_[i] += gamb * n[i];
+        Nb[i] += 0.25 * gamb * n[i];
+    }
+    // shift couplings N, E1, E2, T, L
+    for (int aa = 0; aa < 3; ++aa) {
+        const double* gb = T.gb[aa];
+        double ng = 0, e1g = 0, e2g = 0;
+        for (int j = 0; j < 3; ++j) {
+            ng += n[j] * gb[j];
+            e1g += e1[j] * gb[j];
+            e2g += e2[j] * gb[j];
+        }
+        // N_j = sum_a n_a gb[a][j]
+        for (int j = 0; j < 3; ++j) {
+            nb[aa] += Nb[j] * gb[j];
+            gbb[aa][j] += Nb[j] * n[aa];
+        }
+        // E1 = 1/2 sum_a [e1_a (n.gb_a) + n_a (e1.gb_a)]
+        e1b[aa] += 0.5 * E1b * ng;
+        nb[aa] += 0.5 * E1b * e1g;
+        e2b[aa] += 0.5 * E2b * ng;
+        nb[aa] += 0.5 * E2b * e2g;
+        for (int j = 0; j < 3; ++j) {
+            nb[j] += 0.5 * E1b * e1[aa] * gb[j] + 0.5 * E2b * e2[aa] * gb[j];
+            e1b[j] += 0.5 * E1b * n[aa] * gb[j];
+            e2b[j] += 0.5 * E2b * n[aa] * gb[j];
+            gbb[aa][j] += 0.5 * E1b * (e1[aa] * n[j] + n[aa] * e1[j]) + 0.5 * E2b * (e2[aa] * n[j] + n[aa] * e2[j]);
+        }
+        // L = sum_a n_a tr Hb[a]
+        nb[aa] += Lb * (T.Hb[aa][0][0] + T.Hb[aa][1][1] + T.Hb[aa][2][2]);
+        for (int i = 0; i < 3; ++i) Hbb[aa][i][i] += Lb * n[aa];
+        // T_ij += n_a Hb[a]_ij
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                nb[aa] += Tb[i][j] * T.Hb[aa][i][j];
+                Hbb[aa][i][j] += Tb[i][j] * n[aa];
+            }
+    }
+    // T_ij -= 1/2 [sum_k n_k Hb[i]_kj + sum_k n_k Hb[j]_ki]
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                nb[k] -= 0.5 * Tb[i][j] * (T.Hb[i][k][j] + T.Hb[j][k][i]);
+                Hbb[i][k][j] -= 0.5 * Tb[i][j] * n[k];
+                Hbb[j][k][i] -= 0.5 * Tb[i][j] * n[k];
+            }
+
+    // interpolation adjoints: field f has (value, gradient, Hessian) adjoints
+    double xb[3] = {0, 0, 0};
+    for (int f = 0; f < 4; ++f) {
+        double bar[10];
+        double gcur[3];
+        double Hcur[3][3];
+        if (f == 0) {
+            bar[0] = phib;
+            for (int i = 0; i < 3; ++i) {
+                bar[1 + i] = gb_[i];
+                gcur[i] = T.g[i];
+                for (int j = 0; j < 3; ++j) Hcur[i][j] = T.H[i][j];
+            }
+            bar[4] = Hbar[0][0];
+            bar[5] = Hbar[0][1] + Hbar[1][0];
+            bar[6] = Hbar[0][2] + Hbar[2][0];
+            bar[7] = Hbar[1][1];
+            bar[8] = Hbar[1][2] + Hbar[2][1];
+            bar[9] = Hbar[2][2];
+        } else {
+            const int aa = f - 1;
+            bar[0] = Bvb[aa];
+            for (int i = 0; i < 3; ++i) {
+                bar[1 + i] = gbb[aa][i];
+                gcur[i] = T.gb[aa][i];
+                for (int j = 0; j < 3; ++j) Hcur[i][j] = T.Hb[aa][i][j];
+            }
+            bar[4] = Hbb[aa][0][0];
+            bar[5] = Hbb[aa][0][1] + Hbb[aa][1][0];
+            bar[6] = Hbb[aa][0][2] + Hbb[aa][2][0];
+            bar[7] = Hbb[aa][1][1];
+            bar[8] = Hbb[aa][1][2] + Hbb[aa][2][1];
+            bar[9] = Hbb[aa][2][2];
+        }
+        // position: d(value)/dx = g, d(g_i)/dx_k = H_ik, d(h_u)/dx_k = third derivative
+        static const int hu[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        for (int k = 0; k < 3; ++k) {
+            double acc = bar[0] * gcur[k];
+            for (int i = 0; i < 3; ++i) acc += bar[1 + i] * Hcur[i][k];
+            for (int u = 0; u < 6; ++u) acc += bar[4 + u] * t3[f][t3idx(hu[u][0], hu[u][1], k)];
+            xb[k] += acc;
+        }
+        scatter_taps(gdst[f], st[f].off, (size_t)G.pad, (size_t)G.pad * (size_t)G.pad, st[f].ax, st[f].ay,
+                     st[f].az, bar);
+    }
+
+    // normalisations: n = nd / |nd| etc.
+    double nn = 0, n1 = 0, n2 = 0;
+    for (int i = 0; i < 3; ++i) {
+        nn += n[i] * nb[i];
+        n1 += e1[i] * e1b[i];
+        n2 += e2[i] * e2b[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        sbar[i] = xb[i];
+        sbar[3 + i] = T.rn * (nb[i] - n[i] * nn);
+        sbar[6 + i] = T.r1 * (e1b[i] - e1[i] * n1);
+        sbar[9 + i] = T.r2 * (e2b[i] - e2[i] * n2);
+    }
+    sbar[12] = Db;
+    sbar[13] = Dpb;
+    sbar[14] = s1b;
+    sbar[15] = s2b;
+    sbar[16] = eps1b;
+    sbar[17] = eps2b;
+    sbar[18] = 0.0;
+}
+
+}  // namespace adj
+}  // namespace srt

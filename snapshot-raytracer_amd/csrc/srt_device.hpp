@@ -313,9 +313,17 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
         double t;
         i0[a] = locate(x[a], G.n, G.inv_n, t);
         t0[a] = cell_offset<F>(x[a], t);
-        const double xs = x[a] - 0.5;
-        i1[a] = locate(xs, G.n, G.inv_n, t);
-        t1[a] = cell_offset<F>(xs, t);
+        // the staggered cell follows from the unstaggered one: floor(x - 1/2) is
+        // floor(x) if frac(x) >= 1/2, else floor(x) - 1; the offsets t -/+ 1/2 are
+        // exact for t in [0, 1)
+        const bool upper = t >= 0.5;
+        const int i = upper ? i0[a] : i0[a] - 1;
+        i1[a] = i < 0 ? i + G.n : i;
+        if constexpr (F::kRefOffset) {
+            t1[a] = cell_offset<F>(x[a] - 0.5, upper ? t - 0.5 : t + 0.5);
+        } else {
+            t1[a] = (RI)(upper ? t - 0.5 : t + 0.5);
+        }
     }
     const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
 
@@ -384,7 +392,16 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
     // the FP64 and mixed forms must track the same position to ~1e-10 cells
     {
         const double nn = nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2];
-        const double scale = (1.0 + 2.0 * (double)phi) / sqrt(nn);
+        double rnd;
+        if constexpr (sizeof(RA) == 4) {
+            // FP32 reciprocal square root refined by one Newton step in FP64
+            // (relative error ~1e-14; |n| stays within 1e-3 of 1)
+            const double r0 = (double)__frsqrt_rn((float)nn);
+            rnd = r0 * (1.5 - 0.5 * nn * r0 * r0);
+        } else {
+            rnd = 1.0 / sqrt(nn);
+        }
+        const double scale = (1.0 + 2.0 * (double)phi) * rnd;
         for (int i = 0; i < 3; ++i) {
             kx[i] = -nd[i] * scale - (double)Bv[i];
             dy[i] = (RA)kx[i];

@@ -19,6 +19,7 @@
 #include <string>
 
 #include "../../include/srt.h"
+#include "srt_adjoint.hpp"
 #include "srt_device.hpp"
 
 namespace {
@@ -262,6 +263,208 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
     }
 }
 
+// ------------------------------------------------------------- adjoint ----
+// One ray per lane, FP64.  Forward pass storing the state every K steps
+// (checkpoints), the six maps in FP64, then, segment by segment from the
+// last, recomputation of the segment's step states from its checkpoint and
+// the reverse sweep of each RK4 step (k4, k3, k2, k1 vector-Jacobian
+// products), which scatters dL/d(grid) into padded FP64 gradient grids.
+constexpr int kAdjBlock = 64;
+
+struct AdjArgs {
+    srt::Grids G;
+    int64_t nside, pix_begin, count;
+    const int64_t* pix_list;
+    int n_steps, K;
+    double obs[3];
+    const double* map_grad;  // 6 x count, null = forward only
+    double* maps64;          // 6 x count, may be null
+    double* ckpt;            // count x (nseg + 1) x 19
+    double* work;            // count x (K + 4) x 19
+    double* grad[4];         // padded gradient grids: potential, B_x, B_y, B_z
+};
+
+// FP64 right-hand side on a 19-vector (production kernel arithmetic)
+__device__ __forceinline__ void rhs19(const srt::Grids& G, const double* s, double* k) {
+    double ys[10];
+    for (int j = 0; j < 6; ++j) ys[j] = s[6 + j];
+    for (int j = 0; j < 4; ++j) ys[6 + j] = s[14 + j];
+    double kx[3];
+    srt::rhs<srt::FormFP64>(G, s, s + 3, s[12], s[13], ys, kx, k);
+    for (int i = 0; i < 3; ++i) k[i] = kx[i];
+    k[12] = -s[13];
+}
+
+__device__ __forceinline__ void rk4_step(const srt::Grids& G, double* y) {
+    double t[19], k[19], acc[19];
+    rhs19(G, y, acc);
+    for (int j = 0; j < 19; ++j) t[j] = y[j] + 0.5 * acc[j];
+    rhs19(G, t, k);
+    for (int j = 0; j < 19; ++j) {
+        t[j] = y[j] + 0.5 * k[j];
+        acc[j] += 2.0 * k[j];
+    }
+    rhs19(G, t, k);
+    for (int j = 0; j < 19; ++j) {
+        t[j] = y[j] + k[j];
+        acc[j] += 2.0 * k[j];
+    }
+    rhs19(G, t, k);
+    constexpr double sixth = 1.0 / 6.0;
+    for (int j = 0; j < 19; ++j) y[j] += (acc[j] + k[j]) * sixth;
+}
+
+// dL/ds of one stage given dL/dk, scattering the grid gradient
+__device__ __forceinline__ void stage_vjp(const AdjArgs& A, const double* s, const double* kbar, double* sbar) {
+    srt::adj::Stencil st[4];
+    srt::adj::locate_stencils(A.G, s, st);
+    srt::adj::RhsTape T;
+    double t3[4][10];
+    double dy[19];
+    srt::adj::rhs_forward(A.G, s, st, T, t3, dy);
+    double* const gd[4] = {A.grad[0], A.grad[1], A.grad[2], A.grad[3]};
+    srt::adj::rhs_vjp(A.G, st, T, t3, kbar, gd, sbar);
+}
+
+__global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
+    const int64_t ray = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (ray >= A.count) return;
+    const int64_t pix = A.pix_list ? A.pix_list[ray] : A.pix_begin + ray;
+    const int64_t npix = 12 * A.nside * A.nside;
+    if (pix < 0 || pix >= npix) {
+        if (A.maps64)
+            for (int m = 0; m < 6; ++m) A.maps64[m * A.count + ray] = __builtin_nan("");
+        return;
+    }
+    double z, s, ph;
+    srt::ring_pixel_direction(A.nside, pix, z, s, ph);
+    double sp, cp;
+    sincos(ph, &sp, &cp);
+    double y[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp, 0.0,
+                    0.0,      -1.0,     0.0,      0.0,     0.0,     0.0, 0.0};
+    const int K = A.K;
+    const int nseg = (A.n_steps + K - 1) / K;
+    double* ck = A.ckpt + ray * (int64_t)(nseg + 1) * 19;
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+        for (int j = 0; j < 19; ++j) ck[sgi * 19 + j] = y[j];
+        const int m = min(K, A.n_steps - sgi * K);
+        for (int st = 0; st < m; ++st) rk4_step(A.G, y);
+    }
+    for (int j = 0; j < 19; ++j) ck[nseg * 19 + j] = y[j];
+
+    // maps (.cu:566-582) in FP64, and dL/dy at the end from dL/dmaps
+    const double th[3] = {z * cp, z * sp, -s};
+    const double p2[3] = {sp, -cp, 0.0};
+    double d[3] = {y[0] - A.obs[0], y[1] - A.obs[1], y[2] - A.obs[2]};
+    const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double u[3] = {d[0] / dn, d[1] / dn, d[2] / dn};
+    const double r[3] = {s * cp, s * sp, z};
+    double dl1 = 0, dl2 = 0;
+    for (int i = 0; i < 3; ++i) {
+        dl1 += (u[i] - r[i]) * th[i];
+        dl2 += (u[i] - r[i]) * p2[i];
+    }
+    if (A.maps64) {
+        const int64_t C = A.count;
+        A.maps64[0 * C + ray] = y[12];
+        A.maps64[1 * C + ray] = y[16];
+        A.maps64[2 * C + ray] = y[17];
+        A.maps64[3 * C + ray] = y[18];
+        A.maps64[4 * C + ray] = dl1;
+        A.maps64[5 * C + ray] = dl2;
+    }
+    if (!A.map_grad) return;
+    double lam[19];
+    for (int j = 0; j < 19; ++j) lam[j] = 0.0;
+    const int64_t C = A.count;
+    const double mb[6] = {A.map_grad[0 * C + ray], A.map_grad[1 * C + ray], A.map_grad[2 * C + ray],
+                          A.map_grad[3 * C + ray], A.map_grad[4 * C + ray], A.map_grad[5 * C + ray]};
+    lam[12] = mb[0];
+    lam[16] = mb[1];
+    lam[17] = mb[2];
+    lam[18] = mb[3];
+    {
+        // d(defl)/dx = (1 - u u^T) / |d| applied to mb4 theta-hat + mb5 p2
+        double w[3], uw = 0;
+        for (int i = 0; i < 3; ++i) {
+            w[i] = mb[4] * th[i] + mb[5] * p2[i];
+            uw += u[i] * w[i];
+        }
+        for (int i = 0; i < 3; ++i) lam[i] = (w[i] - u[i] * uw) / dn;
+    }
+
+    double* wk = A.work + ray * (int64_t)(K + 4) * 19;  // K step states, then 4 stage states
+    double* stg = wk + K * 19;
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        for (int j = 0; j < 19; ++j) y[j] = ck[sgi * 19 + j];
+        const int m = min(K, A.n_steps - sgi * K);
+        for (int st = 0; st < m; ++st) {
+            for (int j = 0; j < 19; ++j) wk[st * 19 + j] = y[j];
+            rk4_step(A.G, y);
+        }
+        for (int st = m - 1; st >= 0; --st) {
+            // stage states of this step
+            {
+                double yy[19], k[19];
+                for (int j = 0; j < 19; ++j) yy[j] = wk[st * 19 + j];
+                for (int j = 0; j < 19; ++j) stg[j] = yy[j];
+                rhs19(A.G, yy, k);
+                for (int j = 0; j < 19; ++j) stg[19 + j] = yy[j] + 0.5 * k[j];
+                rhs19(A.G, stg + 19, k);
+                for (int j = 0; j < 19; ++j) stg[38 + j] = yy[j] + 0.5 * k[j];
+                rhs19(A.G, stg + 38, k);
+                for (int j = 0; j < 19; ++j) stg[57 + j] = yy[j] + k[j];
+            }
+            // reverse sweep: y' = y + (k1 + 2 k2 + 2 k3 + k4) / 6
+            double ybar[19], kb[19], sb[19];
+            for (int j = 0; j < 19; ++j) {
+                ybar[j] = lam[j];
+                kb[j] = lam[j] * (1.0 / 6.0);
+            }
+            stage_vjp(A, stg + 57, kb, sb);  // s4 = y + k3
+            for (int j = 0; j < 19; ++j) {
+                ybar[j] += sb[j];
+                kb[j] = sb[j] + lam[j] * (1.0 / 3.0);
+            }
+            stage_vjp(A, stg + 38, kb, sb);  // s3 = y + k2 / 2
+            for (int j = 0; j < 19; ++j) {
+                ybar[j] += sb[j];
+                kb[j] = 0.5 * sb[j] + lam[j] * (1.0 / 3.0);
+            }
+            stage_vjp(A, stg + 19, kb, sb);  // s2 = y + k1 / 2
+            for (int j = 0; j < 19; ++j) {
+                ybar[j] += sb[j];
+                kb[j] = 0.5 * sb[j] + lam[j] * (1.0 / 6.0);
+            }
+            stage_vjp(A, stg, kb, sb);  // s1 = y
+            for (int j = 0; j < 19; ++j) lam[j] = ybar[j] + sb[j];
+        }
+    }
+}
+
+// Folds the padded gradient grid onto the periodic n^3 grid: voxel (i, j, k)
+// collects every padded entry whose index minus one is congruent to it.
+__global__ __launch_bounds__(256) void fold_grad_kernel(const double* __restrict__ padded, int n, int P,
+                                                        double* __restrict__ out, int64_t stride, int comp) {
+    const int64_t total = (int64_t)n * n * n;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < total; v += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(v % n), j = (int)((v / n) % n), k = (int)(v / ((int64_t)n * n));
+        int px[2], py[2], pz[2], cx = 0, cy = 0, cz = 0;
+        // padded index q in [0, P) maps to (q - 1) mod n
+        for (int q = i + 1; q < P; q += n) px[cx++] = q;
+        if (i == n - 1) px[cx++] = 0;
+        for (int q = j + 1; q < P; q += n) py[cy++] = q;
+        if (j == n - 1) py[cy++] = 0;
+        for (int q = k + 1; q < P; q += n) pz[cz++] = q;
+        if (k == n - 1) pz[cz++] = 0;
+        double sum = 0.0;
+        for (int c = 0; c < cz; ++c)
+            for (int b = 0; b < cy; ++b)
+                for (int a = 0; a < cx; ++a) sum += padded[(int64_t)px[a] + (int64_t)P * (py[b] + (int64_t)P * pz[c])];
+        out[v * stride + comp] = sum;
+    }
+}
+
 // LDS bytes per block of trace_kernel<F>
 template <class F>
 constexpr unsigned lds_bytes() {
@@ -278,6 +481,9 @@ struct srt_context {
     float* grids = nullptr;  // 4 padded cubes: phi, Bx, By, Bz
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timed = false;
+    // adjoint work buffers, grown on demand and kept for later calls
+    double* adj_buf = nullptr;
+    size_t adj_bytes = 0;
 };
 
 extern "C" {
@@ -324,6 +530,7 @@ int srt_destroy(srt_context* ctx) {
     if (!ctx) return SRT_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->grids) (void)hipFree(ctx->grids);
+    if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     delete ctx;
@@ -456,6 +663,78 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     }
     SRT_HIP(hipEventRecord(ctx->ev_stop, st));
     ctx->timed = true;
+    return SRT_OK;
+}
+
+int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+                      int n_steps, const double* observer, int checkpoint_every, const double* map_grad,
+                      double* maps64, double* grad_phi, double* grad_shift_xyz, void* stream) {
+    if (!ctx) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: null context");
+    if (!ctx->grids) return fail(SRT_ERR_NOT_UPLOADED, "srt_trace_adjoint: no snapshot uploaded");
+    if (nside <= 0 || nside > (1 << 20)) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: nside out of range");
+    if (n_steps < 1 || checkpoint_every < 1)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: n_steps and checkpoint_every must be >= 1");
+    const int64_t npix = 12 * nside * nside;
+    if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: count out of range");
+    if (!pix_list && (pix_begin < 0 || pix_begin + count > npix))
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: pixel range outside the sky");
+    if (map_grad && (!grad_phi || !grad_shift_xyz))
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: gradient outputs are null");
+    if (!map_grad && !maps64) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: nothing to compute");
+    SRT_HIP(hipSetDevice(ctx->device));
+    const int K = std::min(checkpoint_every, n_steps);
+    const int nseg = (n_steps + K - 1) / K;
+    const size_t P = (size_t)ctx->pad, pvox = P * P * P;
+    const size_t c = (size_t)count;
+    const size_t n_ck = c * (size_t)(nseg + 1) * 19, n_wk = c * (size_t)(K + 4) * 19, n_gr = map_grad ? 4 * pvox : 0;
+    const size_t need = (n_ck + n_wk + n_gr) * sizeof(double);
+    if (need > ctx->adj_bytes) {
+        SRT_HIP(hipStreamSynchronize((hipStream_t)stream));
+        if (ctx->adj_buf) SRT_HIP(hipFree(ctx->adj_buf));
+        ctx->adj_buf = nullptr;
+        ctx->adj_bytes = 0;
+        SRT_HIP(hipMalloc(&ctx->adj_buf, need));
+        ctx->adj_bytes = need;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    AdjArgs A;
+    A.G.phi = ctx->grids;
+    for (int a = 0; a < 3; ++a) A.G.b[a] = ctx->grids + (size_t)(1 + a) * pvox;
+    A.G.n = ctx->n;
+    A.G.pad = ctx->pad;
+    A.G.inv_n = 1.0 / (double)ctx->n;
+    A.nside = nside;
+    A.pix_begin = pix_begin;
+    A.count = count;
+    A.pix_list = pix_list;
+    A.n_steps = n_steps;
+    A.K = K;
+    for (int i = 0; i < 3; ++i) A.obs[i] = observer ? observer[i] : 0.5 * (double)ctx->n;
+    A.map_grad = map_grad;
+    A.maps64 = maps64;
+    A.ckpt = ctx->adj_buf;
+    A.work = ctx->adj_buf + n_ck;
+    for (int f = 0; f < 4; ++f) A.grad[f] = map_grad ? ctx->adj_buf + n_ck + n_wk + f * pvox : nullptr;
+    if (map_grad) SRT_HIP(hipMemsetAsync(A.grad[0], 0, 4 * pvox * sizeof(double), st));
+    SRT_HIP(hipEventRecord(ctx->ev_start, st));
+    if (count > 0) {
+        const int64_t blocks = (count + kAdjBlock - 1) / kAdjBlock;
+        hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, A);
+        SRT_HIP(hipGetLastError());
+    }
+    SRT_HIP(hipEventRecord(ctx->ev_stop, st));
+    ctx->timed = true;
+    if (map_grad) {
+        const int n = ctx->n;
+        const int64_t total = (int64_t)n * n * n;
+        const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(fold_grad_kernel, dim3(blocks), dim3(256), 0, st, A.grad[0], n, (int)P, grad_phi,
+                           (int64_t)1, 0);
+        for (int a = 0; a < 3; ++a)
+            hipLaunchKernelGGL(fold_grad_kernel, dim3(blocks), dim3(256), 0, st, A.grad[1 + a], n, (int)P,
+                               grad_shift_xyz, (int64_t)3, a);
+        SRT_HIP(hipGetLastError());
+    }
     return SRT_OK;
 }
 

@@ -1,0 +1,172 @@
+"""Adjoint kernel (srt_trace_adjoint): gradients of a map loss with respect to
+the potential and shift grids, checked against central finite differences of
+the FP64 forward trace (GPU) and of the FP64 host oracle, in FP64.
+
+Workload: periodic 32^3 snapshot (seed 31), Nside 8 (768 rays), 12 steps,
+checkpoints every 4 steps; loss = sum over pixels of (eps1 - t1)^2 +
+(eps2 - t2)^2 + (dA - tA)^2 with targets from a second seed's maps plus
+Gaussian noise (sigma = 0.01 x map rms).  Finite differences use the step
+1e-3 x field rms on voxels within 8 cells of the observer; the perturbed
+value is the float32 the grid stores, and the difference quotient uses the
+exact float32 step.  Bar: 1e-6 relative to the largest checked gradient.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N, NSIDE, STEPS, K = 32, 8, 12, 4
+
+
+@pytest.fixture(scope="module")
+def setup(srt, fields):
+    import torch
+    phi, shift = fields.snapshot(N, 31)
+    phi2, shift2 = fields.snapshot(N, 32)
+    dev = torch.device("cuda:0")
+    tr = srt.Tracer(0)
+    tr.upload(phi2.numpy(), fields.interleave_shift(shift2).numpy())
+    tmaps, _, _ = tr.trace_adjoint(NSIDE, STEPS)
+    rng = np.random.default_rng(33)
+    sel = [1, 2, 0]  # eps1, eps2, dA rows of the map array
+    target = tmaps.cpu().numpy()[sel]
+    target = target + rng.standard_normal(target.shape) * 0.01 * target.std(axis=1, keepdims=True)
+    return tr, phi.numpy(), shift.numpy(), torch.from_numpy(target).to(dev)
+
+
+def loss_and_grad(maps, target):
+    import torch
+    sel = [1, 2, 0]
+    r = maps[sel] - target
+    L = float((r * r).sum())
+    g = torch.zeros_like(maps)
+    g[sel] = 2 * r
+    return L, g
+
+
+def loss_of(tr, phi, shift_planar, fields, target):
+    import torch
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift_planar, 0, -1)))
+    maps, _, _ = tr.trace_adjoint(NSIDE, STEPS)
+    return loss_and_grad(maps, target)[0]
+
+
+def voxels(rng, count):
+    out = []
+    while len(out) < count:
+        v = rng.integers(0, N, 3)
+        if np.linalg.norm(v - N / 2) < 8:
+            out.append(tuple(int(a) for a in v))
+    return out
+
+
+def test_adjoint_matches_central_differences(srt, setup, fields):
+    import torch
+    tr, phi, shift, target = setup
+    print("[gpu] adjoint gradient", flush=True)
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
+    maps, _, _ = tr.trace_adjoint(NSIDE, STEPS, checkpoint_every=K)
+    L0, g = loss_and_grad(maps, target)
+    _, gphi, gshift = tr.trace_adjoint(NSIDE, STEPS, g, checkpoint_every=K)
+    gphi = gphi.cpu().numpy()
+    gshift = gshift.cpu().numpy()
+    rng = np.random.default_rng(34)
+    checks = []
+    for comp, vs in (("phi", voxels(rng, 8)), (0, voxels(rng, 4)), (1, voxels(rng, 4)), (2, voxels(rng, 4))):
+        base = phi if comp == "phi" else shift[comp]
+        h = 1e-3 * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
+        for (i, j, k) in vs:
+            vals = []
+            for sgn in (+1, -1):
+                p = phi.copy()
+                s = shift.copy()
+                arr = p if comp == "phi" else s[comp]
+                arr[k, j, i] = np.float32(float(base[k, j, i]) + sgn * h)
+                vals.append((loss_of(tr, p, s, fields, target), float(arr[k, j, i])))
+            fd = (vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1])
+            ad = gphi[k, j, i] if comp == "phi" else gshift[k, j, i, comp]
+            checks.append((comp, (i, j, k), fd, ad))
+    scale = max(abs(c[2]) for c in checks)
+    worst = max(abs(c[2] - c[3]) for c in checks) / scale
+    for c in checks:
+        print(f"{c[0]} {c[1]}: fd {c[2]:+.10e} adjoint {c[3]:+.10e}")
+    print(f"worst |fd - adjoint| / max|fd| = {worst:.3e}")
+    assert scale > 0
+    assert worst < 1e-6
+
+
+def test_adjoint_checkpoint_interval_invariance(srt, setup):
+    tr, phi, shift, target = setup
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
+    maps, _, _ = tr.trace_adjoint(NSIDE, STEPS)
+    _, g = loss_and_grad(maps, target)
+    ref = None
+    for k in (1, 5, STEPS):
+        m, gp, gs = tr.trace_adjoint(NSIDE, STEPS, g, checkpoint_every=k)
+        assert np.array_equal(m.cpu().numpy(), maps.cpu().numpy())
+        cur = (gp.cpu().numpy(), gs.cpu().numpy())
+        if ref is None:
+            ref = cur
+        else:  # same arithmetic, only the atomic summation order differs
+            for a, b in zip(cur, ref):
+                np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max())
+
+
+def test_adjoint_forward_maps_match_fp64_kernel(srt, setup):
+    tr, phi, shift, target = setup
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
+    maps64, _, _ = tr.trace_adjoint(NSIDE, STEPS)
+    maps32 = tr.trace(NSIDE, STEPS, precision=srt.PREC_FP64)
+    np.testing.assert_allclose(maps64.cpu().numpy().astype(np.float32), maps32, rtol=3e-7, atol=1e-30)
+
+
+def test_adjoint_vs_oracle_differences(srt, oracle, setup):
+    """The adjoint gradient against central differences of the host FP64 oracle."""
+    tr, phi, shift, target = setup
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
+    maps, _, _ = tr.trace_adjoint(NSIDE, STEPS)
+    _, g = loss_and_grad(maps, target)
+    _, gphi, gshift = tr.trace_adjoint(NSIDE, STEPS, g, checkpoint_every=K)
+    tgt = target.cpu().numpy()
+    pix = np.arange(12 * NSIDE * NSIDE)
+
+    def oracle_loss(p, s):
+        _, _, st = oracle.trace(oracle.Fields(p, s), NSIDE, pix, STEPS, return_state=True)
+        m = np.stack([st[:, 16], st[:, 17], st[:, 12]])
+        return float(((m - tgt) ** 2).sum())
+
+    rng = np.random.default_rng(35)
+    res = []
+    for comp, (i, j, k) in [("phi", v) for v in voxels(rng, 3)] + [(1, v) for v in voxels(rng, 2)]:
+        base = phi if comp == "phi" else shift[comp]
+        h = 1e-3 * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
+        vals = []
+        for sgn in (+1, -1):
+            p = phi.copy()
+            s = shift.copy()
+            arr = p if comp == "phi" else s[comp]
+            arr[k, j, i] = np.float32(float(base[k, j, i]) + sgn * h)
+            vals.append((oracle_loss(p, s), float(arr[k, j, i])))
+        fd = (vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1])
+        ad = float(gphi[k, j, i] if comp == "phi" else gshift[k, j, i, comp])
+        res.append((fd, ad))
+    scale = max(abs(a) for a, _ in res)
+    worst = max(abs(a - b) for a, b in res) / scale
+    print(f"oracle central differences vs adjoint: worst {worst:.3e}")
+    assert worst < 1e-6
+
+
+def test_autograd_wrapper(srt, setup):
+    import torch
+    tr, phi, shift, target = setup
+    dev = torch.device("cuda:0")
+    p = torch.from_numpy(phi).to(dev).double().requires_grad_(True)
+    s = torch.from_numpy(np.ascontiguousarray(np.moveaxis(shift, 0, -1))).to(dev).double().requires_grad_(True)
+    maps = srt.ray_maps(p, s, NSIDE, STEPS, tracer=tr, checkpoint_every=K)
+    loss = ((maps[[1, 2, 0]] - target) ** 2).sum()
+    loss.backward()
+    L, g = loss_and_grad(maps.detach(), target)
+    _, gphi, gshift = tr.trace_adjoint(NSIDE, STEPS, g, checkpoint_every=K)
+    np.testing.assert_allclose(p.grad.cpu().numpy(), gphi.cpu().numpy(), rtol=1e-9, atol=1e-12 * float(gphi.abs().max()))
+    np.testing.assert_allclose(s.grad.cpu().numpy(), gshift.cpu().numpy(), rtol=1e-9,
+                               atol=1e-12 * float(gshift.abs().max()))
