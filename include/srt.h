@@ -140,6 +140,46 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
                       int n_steps, const double* observer, int checkpoint_every, const double* map_grad,
                       double* maps64, double* grad_phi, double* grad_shift_xyz, void* stream);
 
+/* ---- batched persistent launch (no counterpart in the reference, whose unit
+ * of work is one process per snapshot and batch, README.md:41-42) ---- */
+
+/* One instance of a batched trace: a context (one uploaded snapshot), a RING
+ * pixel range, a step count, an observer and a device output of 6 * count
+ * floats (map-major, as srt_trace).  observer_set = 0 means the grid centre. */
+typedef struct {
+    srt_context* ctx;
+    int64_t nside;
+    int64_t pix_begin;
+    int64_t count;
+    int n_steps;
+    int observer_set;
+    double observer[3];
+    float* out;
+} srt_batch_item;
+
+typedef struct srt_plan srt_plan;
+
+/* Builds a plan for tracing all items in one persistent launch: tiles of
+ * consecutive rays of one item are handed to wavefronts by an atomic counter,
+ * so items of different grid size, step count and ray count share the GPU.
+ * All contexts must be on one device and stay alive while the plan is used. */
+int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt_plan** plan);
+int srt_plan_destroy(srt_plan* plan);
+
+/* Tile size in rays (multiple of 64) and number of persistent 256-lane blocks. */
+int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks);
+int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps);
+
+/* Times every (tile_rays, blocks per CU) candidate on `stream` and keeps the
+ * fastest; returns the choice. */
+int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks);
+
+/* Launches the plan on `stream`.  use_graph = 1 captures the launch sequence
+ * into a HIP graph on first use (and after srt_plan_configure) and replays it;
+ * it needs a non-default stream.  Asynchronous; srt_plan_last_ms times it. */
+int srt_plan_launch(srt_plan* plan, int use_graph, void* stream);
+int srt_plan_last_ms(srt_plan* plan, float* ms);
+
 /* Milliseconds between HIP events recorded on the launch stream just before
  * and just after the kernel launches of the last srt_trace call of this
  * context (blocks until that work is done). */

@@ -36,7 +36,16 @@ MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "defl
 
 # C-ABI functions declared in include/srt.h
 EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_trace",
-           "srt_trace_host", "srt_trace_adjoint", "srt_last_trace_ms", "srt_grid_info", "srt_version")
+           "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_configure",
+           "srt_plan_get_config", "srt_plan_autotune", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
+           "srt_grid_info", "srt_version")
+
+
+class BatchItem(ctypes.Structure):
+    """srt_batch_item of include/srt.h."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("nside", ctypes.c_int64), ("pix_begin", ctypes.c_int64),
+                ("count", ctypes.c_int64), ("n_steps", ctypes.c_int), ("observer_set", ctypes.c_int),
+                ("observer", ctypes.c_double * 3), ("out", ctypes.c_void_p)]
 
 _lib = None
 
@@ -68,6 +77,13 @@ def lib():
         L.srt_trace.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P, P]
         L.srt_trace_host.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P]
         L.srt_trace_adjoint.argtypes = [P, i64, i64, i64, P, i32, P, i32, P, P, P, P, P]
+        L.srt_plan_create.argtypes = [P, i32, i32, ctypes.POINTER(P)]
+        L.srt_plan_destroy.argtypes = [P]
+        L.srt_plan_configure.argtypes = [P, i32, i32]
+        L.srt_plan_get_config.argtypes = [P, P, P, P]
+        L.srt_plan_autotune.argtypes = [P, P, P, P]
+        L.srt_plan_launch.argtypes = [P, i32, P]
+        L.srt_plan_last_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.srt_last_trace_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.srt_grid_info.argtypes = [P, P, P, P, P]
         _lib = L
@@ -215,6 +231,69 @@ class Tracer:
         _check(lib().srt_trace_host(self._ctx, nside, start, count, _ptr(pix_arr), n_steps, _ptr(obs),
                                     precision, order, _ptr(out), _ptr(state)), "srt_trace_host")
         return (out, state) if return_state else out
+
+
+class Plan:
+    """A batched persistent launch over many (snapshot, pixel range, steps, observer) instances.
+
+    items: list of dicts with keys tracer (Tracer), nside, pix_begin, count,
+    n_steps, observer (optional) and out (float32 CUDA tensor of 6 * count).
+    """
+
+    def __init__(self, items, precision: int = PREC_MIXED):
+        arr = (BatchItem * len(items))()
+        self._keep = []
+        for i, it in enumerate(items):
+            b = arr[i]
+            b.ctx = it["tracer"]._ctx.value
+            b.nside = it["nside"]
+            b.pix_begin = it.get("pix_begin", 0)
+            b.count = it["count"]
+            b.n_steps = it["n_steps"]
+            obs = it.get("observer")
+            b.observer_set = 0 if obs is None else 1
+            if obs is not None:
+                for k in range(3):
+                    b.observer[k] = float(obs[k])
+            b.out = it["out"].data_ptr()
+            self._keep.append(it["out"])
+        self._plan = ctypes.c_void_p()
+        _check(lib().srt_plan_create(arr, len(items), precision, ctypes.byref(self._plan)), "srt_plan_create")
+
+    def close(self) -> None:
+        if self._plan:
+            lib().srt_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, tile_rays: int, blocks: int) -> None:
+        _check(lib().srt_plan_configure(self._plan, tile_rays, blocks), "srt_plan_configure")
+
+    def config(self):
+        t, b, rs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(lib().srt_plan_get_config(self._plan, ctypes.byref(t), ctypes.byref(b), ctypes.byref(rs)),
+               "srt_plan_get_config")
+        return t.value, b.value, rs.value
+
+    def autotune(self, stream=None):
+        t, b = ctypes.c_int(), ctypes.c_int()
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _check(lib().srt_plan_autotune(self._plan, st, ctypes.byref(t), ctypes.byref(b)), "srt_plan_autotune")
+        return t.value, b.value
+
+    def launch(self, stream=None, graph: bool = False) -> None:
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _check(lib().srt_plan_launch(self._plan, 1 if graph else 0, st), "srt_plan_launch")
+
+    def last_ms(self) -> float:
+        ms = ctypes.c_float()
+        _check(lib().srt_plan_last_ms(self._plan, ctypes.byref(ms)), "srt_plan_last_ms")
+        return float(ms.value)
 
 
 def full_sky_maps(nside: int, batch: int | None, maps: np.ndarray) -> np.ndarray:

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/srt.h"
 #include "srt_adjoint.hpp"
@@ -106,30 +107,15 @@ struct TraceArgs {
     double* state;
 };
 
+// Traces one ray (RING pixel `pix`) and writes its maps to slot `slot` of
+// the map-major output (and the final state if `state` is not null).
 template <class F>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
+__device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, int64_t pix, int64_t slot,
+                                          int64_t count, int n_steps, const double* __restrict__ obs,
+                                          float* __restrict__ out, double* __restrict__ state) {
     using RA = typename F::RA;
-    const int64_t lane_ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (lane_ray >= A.count) return;
-    int64_t pix, slot;
-    if (A.order == SRT_ORDER_NESTED) {
-        pix = srt::nest_to_ring(A.nside, lane_ray);
-        slot = pix;
-    } else {
-        slot = lane_ray;
-        pix = A.pix_list ? A.pix_list[lane_ray] : A.pix_begin + lane_ray;
-    }
-    if (pix < 0 || pix >= 12 * A.nside * A.nside) {
-        // invalid entry of a caller's pixel list: NaN maps in its slot (in NESTED
-        // order the slot is the pixel itself, so nothing can be written)
-        if (A.order != SRT_ORDER_NESTED) {
-            const float nan = __builtin_nanf("");
-            for (int m = 0; m < 6; ++m) A.out[m * A.count + slot] = nan;
-        }
-        return;
-    }
     double z, s, ph;
-    srt::ring_pixel_direction(A.nside, pix, z, s, ph);
+    srt::ring_pixel_direction(nside, pix, z, s, ph);
     double sp, cp;
     sincos(ph, &sp, &cp);
 
@@ -151,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
         else return acc_reg[j];
     };
     {
-        const double init[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp,
+        const double init[19] = {obs[0], obs[1], obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp,
                                  0.0,      0.0,      -1.0,     0.0,      0.0,     0.0, 0.0,    0.0};
 #pragma unroll
         for (int j = 0; j < 19; ++j) y(j) = init[j];
@@ -179,11 +165,11 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
         for (int j = 0; j < 6; ++j) ys[j] = (RA)(y(6 + j) + c * (double)ki[6 + j]);
 #pragma unroll
         for (int j = 0; j < 4; ++j) ys[6 + j] = (RA)(y(14 + j) + c * (double)ki[14 + j]);
-        srt::rhs<F>(A.G, xs, ns, Ds, Dps, ys, kxo, ko);
+        srt::rhs<F>(G, xs, ns, Ds, Dps, ys, kxo, ko);
         kDo = -Dps;
     };
 
-    for (int step = 0; step < A.n_steps; ++step) {
+    for (int step = 0; step < n_steps; ++step) {
         {
             const double zx[3] = {0.0, 0.0, 0.0};
             RA zk[19];
@@ -245,21 +231,91 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
     for (int j = 0; j < 19; ++j) yf[j] = y(j);
 
     // deflection: Delta = unit(x - observer) - r (.cu:566-575)
-    double d[3] = {yf[0] - A.obs[0], yf[1] - A.obs[1], yf[2] - A.obs[2]};
+    double d[3] = {yf[0] - obs[0], yf[1] - obs[1], yf[2] - obs[2]};
     const double rn = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     d[0] = d[0] * rn - s * cp;
     d[1] = d[1] * rn - s * sp;
     d[2] = d[2] * rn - z;
-    const int64_t C = A.count;
-    A.out[0 * C + slot] = (float)yf[12];
-    A.out[1 * C + slot] = (float)yf[16];
-    A.out[2 * C + slot] = (float)yf[17];
-    A.out[3 * C + slot] = (float)yf[18];
-    A.out[4 * C + slot] = (float)(d[0] * z * cp + d[1] * z * sp - d[2] * s);
-    A.out[5 * C + slot] = (float)(d[0] * sp - d[1] * cp);
-    if (A.state) {
+    const int64_t C = count;
+    out[0 * C + slot] = (float)yf[12];
+    out[1 * C + slot] = (float)yf[16];
+    out[2 * C + slot] = (float)yf[17];
+    out[3 * C + slot] = (float)yf[18];
+    out[4 * C + slot] = (float)(d[0] * z * cp + d[1] * z * sp - d[2] * s);
+    out[5 * C + slot] = (float)(d[0] * sp - d[1] * cp);
+    if (state) {
 #pragma unroll
-        for (int j = 0; j < 19; ++j) A.state[slot * 19 + j] = yf[j];
+        for (int j = 0; j < 19; ++j) state[slot * 19 + j] = yf[j];
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
+    const int64_t lane_ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (lane_ray >= A.count) return;
+    int64_t pix, slot;
+    if (A.order == SRT_ORDER_NESTED) {
+        pix = srt::nest_to_ring(A.nside, lane_ray);
+        slot = pix;
+    } else {
+        slot = lane_ray;
+        pix = A.pix_list ? A.pix_list[lane_ray] : A.pix_begin + lane_ray;
+    }
+    if (pix < 0 || pix >= 12 * A.nside * A.nside) {
+        // invalid entry of a caller's pixel list: NaN maps in its slot (in NESTED
+        // order the slot is the pixel itself, so nothing can be written)
+        if (A.order != SRT_ORDER_NESTED) {
+            const float nan = __builtin_nanf("");
+            for (int m = 0; m < 6; ++m) A.out[m * A.count + slot] = nan;
+        }
+        return;
+    }
+    trace_ray<F>(A.G, A.nside, pix, slot, A.count, A.n_steps, A.obs, A.out, A.state);
+}
+
+// ---------------------------------------------------------------- batch ----
+// Many snapshots / observers / pixel ranges / step counts in one persistent
+// launch.  Work items ("tiles") of `tile_rays` consecutive rays of one
+// instance are handed out by an atomic counter, one wavefront at a time, so
+// instances of very different cost (grid size, step count) balance across the
+// chip; every wavefront leaves when the counter passes the last tile.
+struct BatchInst {
+    srt::Grids G;
+    int64_t nside, pix_begin, count;
+    int n_steps;
+    double obs[3];
+    float* out;
+    int64_t tile_begin;  // first tile index of this instance
+};
+
+__global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0ull; }
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInst* __restrict__ inst, int n_inst,
+                                                       int64_t total_tiles, int tile_rays,
+                                                       unsigned long long* __restrict__ counter) {
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1ull);
+        const uint32_t lo32 = __builtin_amdgcn_readfirstlane((uint32_t)t);
+        const uint32_t hi32 = __builtin_amdgcn_readfirstlane((uint32_t)(t >> 32));
+        const int64_t tile = (int64_t)(((uint64_t)hi32 << 32) | lo32);
+        if (tile >= total_tiles) break;
+        int lo = 0, hi = n_inst - 1;  // last instance with tile_begin <= tile
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (inst[mid].tile_begin <= tile)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchInst& I = inst[lo];
+        const int64_t r0 = (tile - I.tile_begin) * tile_rays;
+        for (int c = 0; c < tile_rays; c += 64) {
+            const int64_t r = r0 + c + lane;
+            if (r < I.count) trace_ray<F>(I.G, I.nside, I.pix_begin + r, r, I.count, I.n_steps, I.obs, I.out, nullptr);
+        }
     }
 }
 
@@ -735,6 +791,247 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
                                grad_shift_xyz, (int64_t)3, a);
         SRT_HIP(hipGetLastError());
     }
+    return SRT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batch plans ----
+struct srt_plan {
+    int device = 0;
+    int precision = SRT_PREC_MIXED;
+    int n_inst = 0;
+    int64_t total_rays = 0, total_ray_steps = 0;
+    std::vector<srt_batch_item> items;
+    BatchInst* d_inst = nullptr;
+    unsigned long long* d_counter = nullptr;
+    int tile_rays = 256;
+    int blocks = 0;
+    int64_t total_tiles = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timed = false;
+};
+
+namespace {
+
+void plan_drop_graph(srt_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+    p->graph_stream = nullptr;
+}
+
+// (re)builds the device instance table for the plan's tile size
+int plan_build_table(srt_plan* p) {
+    std::vector<BatchInst> h(p->n_inst);
+    int64_t tiles = 0;
+    for (int i = 0; i < p->n_inst; ++i) {
+        const srt_batch_item& it = p->items[i];
+        srt_context* c = it.ctx;
+        BatchInst& b = h[i];
+        const size_t pvox = (size_t)c->pad * c->pad * c->pad;
+        b.G.phi = c->grids;
+        for (int a = 0; a < 3; ++a) b.G.b[a] = c->grids + (size_t)(1 + a) * pvox;
+        b.G.n = c->n;
+        b.G.pad = c->pad;
+        b.G.inv_n = 1.0 / (double)c->n;
+        b.nside = it.nside;
+        b.pix_begin = it.pix_begin;
+        b.count = it.count;
+        b.n_steps = it.n_steps;
+        for (int k = 0; k < 3; ++k) b.obs[k] = it.observer_set ? it.observer[k] : 0.5 * (double)c->n;
+        b.out = it.out;
+        b.tile_begin = tiles;
+        tiles += (it.count + p->tile_rays - 1) / p->tile_rays;
+    }
+    p->total_tiles = tiles;
+    SRT_HIP(hipSetDevice(p->device));
+    SRT_HIP(hipMemcpy(p->d_inst, h.data(), h.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+    return SRT_OK;
+}
+
+int plan_enqueue(srt_plan* p, hipStream_t st) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p->d_counter);
+    switch (p->precision) {
+        case SRT_PREC_FP64:
+            hipLaunchKernelGGL(batch_kernel<srt::FormFP64>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormFP64>(),
+                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->d_counter);
+            break;
+        case SRT_PREC_MIXED:
+            hipLaunchKernelGGL(batch_kernel<srt::FormMixed>, dim3(p->blocks), dim3(kBlock),
+                               lds_bytes<srt::FormMixed>(), st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays,
+                               p->d_counter);
+            break;
+        default:
+            hipLaunchKernelGGL(batch_kernel<srt::FormRef>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormRef>(),
+                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->d_counter);
+            break;
+    }
+    SRT_HIP(hipGetLastError());
+    return SRT_OK;
+}
+
+int device_cus(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 256;
+    return prop.multiProcessorCount;
+}
+
+}  // namespace
+
+extern "C" {
+
+int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt_plan** plan) {
+    if (!plan || !items || n_items <= 0) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: bad arguments");
+    *plan = nullptr;
+    if (precision != SRT_PREC_FP64 && precision != SRT_PREC_MIXED && precision != SRT_PREC_REF)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: unknown precision");
+    const int device = items[0].ctx ? items[0].ctx->device : -1;
+    for (int i = 0; i < n_items; ++i) {
+        const srt_batch_item& it = items[i];
+        if (!it.ctx || !it.out) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: null context or output");
+        if (!it.ctx->grids) return fail(SRT_ERR_NOT_UPLOADED, "srt_plan_create: context without snapshot");
+        if (it.ctx->device != device)
+            return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: contexts on different devices");
+        if (it.nside <= 0 || it.nside > (1 << 20) || it.n_steps < 0 || it.count < 0 || it.pix_begin < 0 ||
+            it.pix_begin + it.count > 12 * it.nside * it.nside)
+            return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: item " + std::to_string(i) + " out of range");
+    }
+    SRT_HIP(hipSetDevice(device));
+    srt_plan* p = new srt_plan();
+    p->device = device;
+    p->precision = precision;
+    p->n_inst = n_items;
+    p->items.assign(items, items + n_items);
+    for (const auto& it : p->items) {
+        p->total_rays += it.count;
+        p->total_ray_steps += it.count * (int64_t)it.n_steps;
+    }
+    p->blocks = 2 * device_cus(device);
+    hipError_t e = hipMalloc(&p->d_inst, sizeof(BatchInst) * n_items);
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipEventCreate(&p->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev_stop);
+    if (e != hipSuccess) {
+        srt_plan_destroy(p);
+        return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
+                    std::string("srt_plan_create: ") + hipGetErrorName(e));
+    }
+    const int rc = plan_build_table(p);
+    if (rc != SRT_OK) {
+        srt_plan_destroy(p);
+        return rc;
+    }
+    *plan = p;
+    return SRT_OK;
+}
+
+int srt_plan_destroy(srt_plan* plan) {
+    if (!plan) return SRT_OK;
+    (void)hipSetDevice(plan->device);
+    plan_drop_graph(plan);
+    if (plan->d_inst) (void)hipFree(plan->d_inst);
+    if (plan->d_counter) (void)hipFree(plan->d_counter);
+    if (plan->ev_start) (void)hipEventDestroy(plan->ev_start);
+    if (plan->ev_stop) (void)hipEventDestroy(plan->ev_stop);
+    delete plan;
+    return SRT_OK;
+}
+
+int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_configure: null plan");
+    if (tile_rays < 64 || tile_rays > 65536 || (tile_rays & 63) || blocks < 1 || blocks > (1 << 20))
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_configure: tile_rays must be a multiple of 64 in "
+                                              "[64, 65536] and blocks in [1, 2^20]");
+    SRT_HIP(hipSetDevice(plan->device));
+    SRT_HIP(hipDeviceSynchronize());
+    plan_drop_graph(plan);
+    plan->tile_rays = tile_rays;
+    plan->blocks = blocks;
+    return plan_build_table(plan);
+}
+
+int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_get_config: null plan");
+    if (tile_rays) *tile_rays = plan->tile_rays;
+    if (blocks) *blocks = plan->blocks;
+    if (total_ray_steps) *total_ray_steps = plan->total_ray_steps;
+    return SRT_OK;
+}
+
+int srt_plan_launch(srt_plan* plan, int use_graph, void* stream) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_launch: null plan");
+    SRT_HIP(hipSetDevice(plan->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (use_graph && !st)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_launch: graph replay needs a non-default stream");
+    if (use_graph && plan->exec && plan->graph_stream != st) plan_drop_graph(plan);
+    if (use_graph && !plan->exec) {
+        SRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = plan_enqueue(plan, st);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc != SRT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        SRT_HIP(e);
+        plan->graph = g;
+        SRT_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+        plan->graph_stream = st;
+    }
+    SRT_HIP(hipEventRecord(plan->ev_start, st));
+    if (use_graph) {
+        SRT_HIP(hipGraphLaunch(plan->exec, st));
+    } else {
+        const int rc = plan_enqueue(plan, st);
+        if (rc != SRT_OK) return rc;
+    }
+    SRT_HIP(hipEventRecord(plan->ev_stop, st));
+    plan->timed = true;
+    return SRT_OK;
+}
+
+int srt_plan_last_ms(srt_plan* plan, float* ms) {
+    if (!plan || !ms) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_last_ms: null pointer");
+    if (!plan->timed) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_last_ms: no launch recorded");
+    SRT_HIP(hipSetDevice(plan->device));
+    SRT_HIP(hipEventSynchronize(plan->ev_stop));
+    SRT_HIP(hipEventElapsedTime(ms, plan->ev_start, plan->ev_stop));
+    return SRT_OK;
+}
+
+int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_autotune: null plan");
+    const int cus = device_cus(plan->device);
+    const int tiles[] = {64, 128, 256, 512, 1024};
+    const int per_cu[] = {1, 2, 4};
+    float best = 0.0f;
+    int bt = plan->tile_rays, bb = plan->blocks;
+    bool have = false;
+    for (int t : tiles)
+        for (int w : per_cu) {
+            int rc = srt_plan_configure(plan, t, w * cus);
+            if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);  // warm-up
+            float ms = 0.0f;
+            if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);
+            if (rc == SRT_OK) rc = srt_plan_last_ms(plan, &ms);
+            if (rc != SRT_OK) return rc;
+            if (!have || ms < best) {
+                best = ms;
+                bt = t;
+                bb = w * cus;
+                have = true;
+            }
+        }
+    const int rc = srt_plan_configure(plan, bt, bb);
+    if (rc != SRT_OK) return rc;
+    if (tile_rays) *tile_rays = bt;
+    if (blocks) *blocks = bb;
     return SRT_OK;
 }
 

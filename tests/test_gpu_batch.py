@@ -1,0 +1,106 @@
+"""Batched persistent launch (srt_plan_*): many snapshots with different grid
+sizes, observers, pixel ranges and step counts in one launch, direct or
+replayed from a HIP graph, with an autotuned tile size and grid.  Every
+instance must equal its own single-instance srt_trace bit for bit (the same
+per-ray code runs in both), on every replay."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def instances(srt, fields):
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(41)
+    out = []
+    for i, n in enumerate([16, 24, 32, 20, 12, 28, 32]):
+        phi, shift = fields.snapshot(n, 100 + 4 * i)
+        tr = srt.Tracer(0)
+        tr.upload(phi.to(dev), fields.interleave_shift(shift).to(dev))
+        nside = int(rng.choice([2, 4, 8]))
+        b = int(rng.integers(0, 12))
+        lo, cnt = srt.batch_range(nside, b)
+        n_steps = int(rng.integers(4, n // 2 + 1))
+        obs = rng.uniform(0, n, 3)
+        out.append(dict(tracer=tr, nside=nside, pix_begin=lo, count=cnt, n_steps=n_steps, observer=obs,
+                        out=torch.zeros(6 * cnt, dtype=torch.float32, device=dev)))
+    return out
+
+
+def expected(srt, items, precision):
+    import torch
+    res = []
+    for it in items:
+        o = torch.empty_like(it["out"])
+        it["tracer"].trace_device(it["nside"], it["n_steps"], o, pix_begin=it["pix_begin"], count=it["count"],
+                                  observer=it["observer"], precision=precision)
+        res.append(o)
+    torch.cuda.synchronize()
+    return [r.cpu().numpy() for r in res]
+
+
+def check(items, ref):
+    for it, r in zip(items, ref):
+        assert np.array_equal(it["out"].cpu().numpy(), r)
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_plan_direct_and_graph_replay(srt, instances, prec):
+    import torch
+    print(f"[gpu] batch plan {prec}", flush=True)
+    p = srt.PRECISIONS[prec]
+    ref = expected(srt, instances, p)
+    plan = srt.Plan(instances, precision=p)
+    plan.launch()
+    torch.cuda.synchronize()
+    check(instances, ref)
+    s = torch.cuda.Stream()
+    for rep in range(3):
+        for it in instances:  # eager work between replays
+            it["out"].zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            plan.launch(stream=s, graph=True)
+        s.synchronize()
+        check(instances, ref)
+    for tile, blocks in ((64, 1), (128, 7), (1024, 600)):
+        plan.configure(tile, blocks)
+        for it in instances:
+            it["out"].zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            plan.launch(stream=s, graph=True)
+        s.synchronize()
+        check(instances, ref)
+    plan.close()
+
+
+def test_plan_autotune(srt, instances):
+    import torch
+    ref = expected(srt, instances, srt.PREC_MIXED)
+    plan = srt.Plan(instances, precision=srt.PREC_MIXED)
+    tile, blocks = plan.autotune()
+    assert tile % 64 == 0 and blocks > 0
+    assert plan.config()[:2] == (tile, blocks)
+    assert plan.config()[2] == sum(it["count"] * it["n_steps"] for it in instances)
+    for it in instances:
+        it["out"].zero_()
+    plan.launch()
+    torch.cuda.synchronize()
+    check(instances, ref)
+    assert plan.last_ms() > 0
+    plan.close()
+
+
+def test_plan_errors(srt, instances):
+    import torch
+    with pytest.raises(srt.SrtError):
+        srt.Plan([dict(instances[0], count=10 ** 9)])
+    plan = srt.Plan(instances[:2])
+    with pytest.raises(srt.SrtError):
+        plan.configure(100, 4)  # not a multiple of 64
+    with pytest.raises(srt.SrtError):
+        plan.launch(graph=True)  # graph replay needs a non-default stream
+    plan.close()
