@@ -24,13 +24,20 @@ sys.path.insert(0, ROOT)
 
 import __graft_entry__  # noqa: E402
 
-# Executed floating-point operations per ray-step of each kernel (FMA = 2),
-# from the rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} counters of
-# the 64^3 / Nside 64 / 64-step profile committed under profiles/.
-FLOPS_PER_RAY_STEP = {"fp64": None, "mixed": None}
+# Floating-point operations per ray-step executed by each kernel (FMA = 2),
+# from rocprofv3 counters SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP64
+# (per wavefront instruction, x 64 lanes) of one launch of this workload,
+# divided by its 3,221,225,472 ray-steps (profiles/trace_kernel_512cube_nside1024_counters.md).
+FLOPS_PER_RAY_STEP = {"mixed": 12730.0, "fp64": 13038.0}
+# HBM bytes per launch of the mixed kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (the counters scaled by what they read for a 1 GiB device copy), same profile.
+HBM_BYTES_PER_LAUNCH = {"mixed": 16.5e9}
 # The reference algorithm's count (oracle with an operation-counting scalar).
 REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
+# MI355X published vector peaks (FP32 counts packed FP32; this build disables
+# packed FP32, so 78.6 TFLOP/s is the attainable FP32 ceiling).
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}
+UNPACKED_FP32_TFLOPS = 78.6
 
 
 def parse():
@@ -47,6 +54,11 @@ def parse():
     p.add_argument("--subset-frac", type=float, default=0.01)
     p.add_argument("--no-oracle", action="store_true", help="skip the CPU oracle leg (accuracy + cpu baseline)")
     p.add_argument("--cpu-threads", type=int, default=16)
+    p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch"],
+                   help="headline: 512^3 / Nside 1024 / 256 steps (the JSON line the driver reads); "
+                        "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
+                        "Nside 64 / 64 steps gradient with finite-difference check; batch: 48 uneven "
+                        "instances in one persistent launch")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -67,6 +79,172 @@ def map_errors(test, ref):
         out[name + "_rms_rel"] = float(np.sqrt(np.mean(d ** 2)) / rms)
         out[name + "_max_rel"] = float(np.max(np.abs(d)) / rms)
     return out
+
+
+def synth(fields, n, seed, dev):
+    phi, shift = fields.snapshot(n, seed, device=dev)
+    return phi, fields.interleave_shift(shift), shift
+
+
+def workload_large(args, pkg, fields, dev):
+    """1024^3 potential + 3 x 1024^3 shift, Nside 2048 full sky, 512 steps, mixed, one launch."""
+    import torch
+    n, nside, n_steps = 1024, 2048, 512
+    rays = 12 * nside * nside
+    t0 = time.time()
+    tr = pkg.Tracer(0)
+    phi, shift_il, shift = synth(fields, n, 3, dev)
+    del shift
+    tr.upload(phi, shift_il)
+    del phi, shift_il
+    torch.cuda.empty_cache()
+    print(f"[bench] large: snapshot + upload {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = torch.empty(6 * rays, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {}
+    for order_name, order in (("nested", pkg.ORDER_NESTED), ("linear", pkg.ORDER_LINEAR)):
+        tr.trace_device(nside, n_steps, out, precision=pkg.PREC_MIXED, order=order, stream=stream)
+        ms = tr.last_trace_ms()
+        res[order_name] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0)}
+        print(f"[bench] large {order_name}: {ms:.0f} ms, {res[order_name]['ray_steps_per_s']:.3e} ray-steps/s",
+              file=sys.stderr, flush=True)
+    print(json.dumps({"workload": "large", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
+                                                      "rays": rays, "precision": "mixed"}, "results": res}))
+
+
+def workload_adjoint(args, pkg, fields, dev):
+    """128^3, Nside 64, 64 steps, checkpoints every 16; loss vs a second seed's maps + noise;
+    central differences on 64 voxels within 60 cells of the observer."""
+    import numpy as np
+    import torch
+    n, nside, n_steps, K = 128, 64, 64, 16
+    count = 12 * nside * nside
+    tr = pkg.Tracer(0)
+    phi_t, shift_t, _ = synth(fields, n, 5, dev)
+    tr.upload(phi_t, shift_t)
+    tmaps, _, _ = tr.trace_adjoint(nside, n_steps)
+    rng = np.random.default_rng(6)
+    sel = [1, 2, 0]
+    target = tmaps[sel] + torch.from_numpy(rng.standard_normal((3, count)) * 0.01).to(dev)
+    phi, shift_il, _ = synth(fields, n, 4, dev)
+    phi_h, shift_h = phi.cpu().numpy(), shift_il.cpu().numpy()
+
+    def loss(maps):
+        r = maps[sel] - target
+        return float((r * r).sum()), r
+
+    tr.upload(phi_h, shift_h)
+    maps, _, _ = tr.trace_adjoint(nside, n_steps)
+    L0, r = loss(maps)
+    g = torch.zeros_like(maps)
+    g[sel] = 2 * r
+    tr.trace_adjoint(nside, n_steps, g, checkpoint_every=K)  # warm-up
+    t0 = time.perf_counter()
+    _, gphi, gshift = tr.trace_adjoint(nside, n_steps, g, checkpoint_every=K)
+    torch.cuda.synchronize()
+    adj_s = time.perf_counter() - t0
+    adj_kernel_ms = tr.last_trace_ms()
+    gphi, gshift = gphi.cpu().numpy(), gshift.cpu().numpy()
+    rng = np.random.default_rng(9)
+    checks = []
+    while len(checks) < 64:
+        v = rng.integers(0, n, 3)
+        if np.linalg.norm(v - n / 2) >= 60:
+            continue
+        i, j, k = (int(a) for a in v)
+        comp = len(checks) % 4  # 0 = potential, 1..3 = shift components
+        base = phi_h if comp == 0 else shift_h[..., comp - 1]
+        h = 1e-3 * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
+        vals = []
+        for sgn in (1, -1):
+            p2, s2 = phi_h.copy(), shift_h.copy()
+            if comp == 0:
+                p2[k, j, i] = np.float32(float(phi_h[k, j, i]) + sgn * h)
+                step = float(p2[k, j, i])
+            else:
+                s2[k, j, i, comp - 1] = np.float32(float(shift_h[k, j, i, comp - 1]) + sgn * h)
+                step = float(s2[k, j, i, comp - 1])
+            tr.upload(p2, s2)
+            m, _, _ = tr.trace_adjoint(nside, n_steps)
+            vals.append((m[sel], step))
+        # L+ - L- = sum (m+ - m-) (m+ + m- - 2 t): no cancellation between two large sums
+        dm = vals[0][0] - vals[1][0]
+        dL = float((dm * (vals[0][0] + vals[1][0] - 2 * target)).sum())
+        fd = dL / (vals[0][1] - vals[1][1])
+        ad = float(gphi[k, j, i] if comp == 0 else gshift[k, j, i, comp - 1])
+        checks.append((comp, fd, ad))
+    scale = {c: max(abs(x[1]) for x in checks if x[0] == c) for c in range(4)}
+    worst = {c: max(abs(x[1] - x[2]) for x in checks if x[0] == c) / scale[c] for c in range(4)}
+    print(json.dumps({"workload": "adjoint", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
+                                                        "checkpoint_every": K, "rays": count},
+                      "adjoint_wall_s": adj_s, "adjoint_kernel_ms": adj_kernel_ms,
+                      "ray_steps_per_s": count * n_steps / adj_s,
+                      "fd_voxels": len(checks),
+                      "max_abs_diff_over_max_fd": {"potential": worst[0], "shift_x": worst[1], "shift_y": worst[2],
+                                                   "shift_z": worst[3]}}))
+
+
+def workload_batch(args, pkg, fields, dev):
+    """48 instances, grids from {128,192,256,384}^3, observers uniform in the box, n_steps in
+    [64, N/2], Nside from {128,256,512}, one random RING batch each (rng 7, seeds 1000 + 4 i)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(7)
+    items = []
+    t0 = time.time()
+    for i in range(48):
+        n = int(rng.choice([128, 192, 256, 384]))
+        nside = int(rng.choice([128, 256, 512]))
+        b = int(rng.integers(0, 12))
+        n_steps = int(rng.integers(64, n // 2 + 1))
+        obs = rng.uniform(0, n, 3)
+        tr = pkg.Tracer(0)
+        phi, shift_il, _ = synth(fields, n, 1000 + 4 * i, dev)
+        tr.upload(phi, shift_il)
+        del phi, shift_il
+        lo, cnt = pkg.batch_range(nside, b)
+        items.append(dict(tracer=tr, nside=nside, pix_begin=lo, count=cnt, n_steps=n_steps, observer=obs,
+                          out=torch.empty(6 * cnt, dtype=torch.float32, device=dev)))
+    torch.cuda.synchronize()
+    print(f"[bench] batch: 48 snapshots in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    total = sum(it["count"] * it["n_steps"] for it in items)
+    stream = torch.cuda.Stream()
+    # one launch per instance, back to back
+    with torch.cuda.stream(stream):
+        for it in items:
+            it["tracer"].trace_device(it["nside"], it["n_steps"], it["out"], pix_begin=it["pix_begin"],
+                                      count=it["count"], observer=it["observer"], precision=pkg.PREC_MIXED,
+                                      stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for it in items:
+            it["tracer"].trace_device(it["nside"], it["n_steps"], it["out"], pix_begin=it["pix_begin"],
+                                      count=it["count"], observer=it["observer"], precision=pkg.PREC_MIXED,
+                                      stream=stream)
+        stream.synchronize()
+    seq_s = time.perf_counter() - t0
+    plan = pkg.Plan(items, precision=pkg.PREC_MIXED)
+    default_cfg = plan.config()[:2]
+    plan.launch(stream=stream)
+    stream.synchronize()
+    default_ms = plan.last_ms()
+    tile, blocks = plan.autotune(stream=stream)
+    res = {}
+    for graph in (False, True):
+        plan.launch(stream=stream, graph=graph)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.launch(stream=stream, graph=graph)
+        stream.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        res["graph" if graph else "direct"] = {"ms": wall * 1000, "ray_steps_per_s": total / wall,
+                                               "kernel_ms": plan.last_ms()}
+    print(json.dumps({"workload": "batch", "instances": 48, "ray_steps": total,
+                      "sequential_launches": {"ms": seq_s * 1000, "ray_steps_per_s": total / seq_s},
+                      "plan_default": {"tile_rays": default_cfg[0], "blocks": default_cfg[1], "ms": default_ms},
+                      "autotuned": {"tile_rays": tile, "blocks": blocks}, "plan": res}))
+    plan.close()
 
 
 def main():
@@ -90,6 +268,10 @@ def main():
         torch.cuda.synchronize()
         print("[bench] calibration copy: 1 GiB read + 1 GiB written", file=sys.stderr, flush=True)
         del src, dst
+    if args.workload != "headline":
+        {"large": workload_large, "adjoint": workload_adjoint, "batch": workload_batch}[args.workload](
+            args, pkg, fields, dev)
+        return
     n, nside, n_steps = args.grid, args.nside, args.n_steps
     rays = 12 * nside * nside
     t0 = time.time()
@@ -146,10 +328,16 @@ def main():
         print(f"[bench] oracle fp64 on {subset.size} rays x {n_steps} steps, {threads} threads",
               file=sys.stderr, flush=True)
         ref, sec = oracle.trace(Fh, nside, subset, n_steps, form=oracle.FORM_FP64, nthreads=threads)
-        cpu_baseline = {"value": subset.size * n_steps / sec, "unit": "ray-steps/s", "cores": threads,
+        # a further disjoint timed sample brings the CPU work to ~10-20 s
+        extra = np.setdiff1d(np.sort(np.random.default_rng(10).choice(rays, 2 * subset.size, replace=False)),
+                             subset)[: int(1.5 * subset.size)]
+        _, sec2 = oracle.trace(Fh, nside, extra, n_steps, form=oracle.FORM_FP64, nthreads=threads)
+        nr = subset.size + extra.size
+        cpu_baseline = {"value": nr * n_steps / (sec + sec2), "unit": "ray-steps/s", "cores": threads,
                         "kind": "port",
-                        "sample": f"FP64 host oracle (OpenMP C++) on the seeded 1% pixel subset "
-                                  f"({subset.size} rays x {n_steps} steps) of the same snapshot, {sec:.1f} s"}
+                        "sample": f"FP64 host oracle (OpenMP C++, {threads} threads) on {nr} seeded pixels "
+                                  f"(the 1% accuracy subset plus {extra.size} more) x {n_steps} steps of the same "
+                                  f"snapshot, {sec + sec2:.1f} s"}
         for name, m in maps_sub.items():
             accuracy[name] = map_errors(m, ref)
 
@@ -159,10 +347,16 @@ def main():
     roofline = None
     if flops:
         achieved = flops * rays * n_steps / (r["kernel_ms_avg"] / 1000.0) / 1e12
-        roofline = {"bound": "mfma", "compute_unit": "VALU (no matrix work on this path)",
+        roofline = {"bound": "mfma", "compute_unit": "VALU vector FP32/FP64 (nothing on this path is a "
+                                                      "matrix product)",
                     "achieved": achieved, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
-                    "frac": achieved / PEAK_TFLOPS[head], "traffic": None,
-                    "flops_per_ray_step": flops, "kernel": f"trace_kernel<{head}>"}
+                    "frac": achieved / PEAK_TFLOPS[head],
+                    "frac_of_unpacked_fp32_peak": achieved / UNPACKED_FP32_TFLOPS,
+                    "traffic": HBM_BYTES_PER_LAUNCH.get(head),
+                    "hbm_frac": (HBM_BYTES_PER_LAUNCH[head] / (r["kernel_ms_avg"] / 1000.0) / 8e12
+                                 if head in HBM_BYTES_PER_LAUNCH else None),
+                    "flops_per_ray_step": flops, "reference_algorithm_flops_per_ray_step":
+                        REF_ALGO_FLOPS_PER_RAY_STEP, "kernel": f"trace_kernel<{head}>"}
     line = {
         "metric": "ray-steps/s at 512^3, Nside 1024, 256 steps (mixed-precision kernel)",
         "value": r["ray_steps_per_s"],

@@ -93,6 +93,12 @@ __global__ __launch_bounds__(256) void pad_grid_kernel(const float* __restrict__
     }
 }
 
+__global__ __launch_bounds__(256) void widen_kernel(const float* __restrict__ src, double* __restrict__ dst,
+                                                    int64_t count) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x)
+        dst[q] = (double)src[q];
+}
+
 // ---------------------------------------------------------------- trace ----
 struct TraceArgs {
     srt::Grids G;
@@ -540,7 +546,30 @@ struct srt_context {
     // adjoint work buffers, grown on demand and kept for later calls
     double* adj_buf = nullptr;
     size_t adj_bytes = 0;
+    // FP64 copies of the padded grids, built on the first FP64 trace
+    double* grids64 = nullptr;
 };
+
+namespace {
+
+// FP64 copies of the padded grids (the FP64 kernel and the adjoint read them)
+int ensure_grids64(srt_context* ctx, hipStream_t st) {
+    if (ctx->grids64) return SRT_OK;
+    const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
+    SRT_HIP(hipMalloc(&ctx->grids64, 4 * pvox * sizeof(double)));
+    const int blocks = (int)std::min<int64_t>(((int64_t)(4 * pvox) + 255) / 256, 65536);
+    hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, st, ctx->grids, ctx->grids64, (int64_t)(4 * pvox));
+    SRT_HIP(hipGetLastError());
+    return SRT_OK;
+}
+
+void set_grids64(const srt_context* ctx, srt::Grids& G) {
+    const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
+    G.phi64 = ctx->grids64;
+    for (int a = 0; a < 3; ++a) G.b64[a] = ctx->grids64 ? ctx->grids64 + (size_t)(1 + a) * pvox : nullptr;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -587,6 +616,7 @@ int srt_destroy(srt_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->grids) (void)hipFree(ctx->grids);
     if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
+    if (ctx->grids64) (void)hipFree(ctx->grids64);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     delete ctx;
@@ -619,6 +649,10 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
         SRT_HIP(hipFree(ctx->grids));
         ctx->grids = nullptr;
         ctx->n = ctx->pad = 0;
+    }
+    if (ctx->grids64) {
+        SRT_HIP(hipFree(ctx->grids64));
+        ctx->grids64 = nullptr;
     }
     float* grids = nullptr;
     SRT_HIP(hipMalloc(&grids, 4 * pvox * sizeof(float)));
@@ -683,6 +717,12 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     A.G.phi = ctx->grids;
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
     for (int a = 0; a < 3; ++a) A.G.b[a] = ctx->grids + (size_t)(1 + a) * pvox;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == SRT_PREC_FP64) {
+        const int rc = ensure_grids64(ctx, st);
+        if (rc != SRT_OK) return rc;
+    }
+    set_grids64(ctx, A.G);
     A.G.n = ctx->n;
     A.G.pad = ctx->pad;
     A.G.inv_n = 1.0 / (double)ctx->n;
@@ -695,7 +735,6 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     for (int i = 0; i < 3; ++i) A.obs[i] = observer ? observer[i] : 0.5 * (double)ctx->n;
     A.out = out;
     A.state = final_state;
-    hipStream_t st = (hipStream_t)stream;
     SRT_HIP(hipEventRecord(ctx->ev_start, st));
     if (count > 0) {
         const int64_t blocks = (count + kBlock - 1) / kBlock;
@@ -759,6 +798,11 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     A.G.n = ctx->n;
     A.G.pad = ctx->pad;
     A.G.inv_n = 1.0 / (double)ctx->n;
+    {
+        const int rc = ensure_grids64(ctx, st);
+        if (rc != SRT_OK) return rc;
+    }
+    set_grids64(ctx, A.G);
     A.nside = nside;
     A.pix_begin = pix_begin;
     A.count = count;
@@ -839,6 +883,11 @@ int plan_build_table(srt_plan* p) {
         b.G.n = c->n;
         b.G.pad = c->pad;
         b.G.inv_n = 1.0 / (double)c->n;
+        if (p->precision == SRT_PREC_FP64) {
+            const int rc = ensure_grids64(c, nullptr);
+            if (rc != SRT_OK) return rc;
+        }
+        set_grids64(c, b.G);
         b.nside = it.nside;
         b.pix_begin = it.pix_begin;
         b.count = it.count;
@@ -850,6 +899,7 @@ int plan_build_table(srt_plan* p) {
     }
     p->total_tiles = tiles;
     SRT_HIP(hipSetDevice(p->device));
+    SRT_HIP(hipDeviceSynchronize());
     SRT_HIP(hipMemcpy(p->d_inst, h.data(), h.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
     return SRT_OK;
 }
