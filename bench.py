@@ -7,8 +7,9 @@ the Nyquist wavenumber, seed 2), observer at the grid centre, HEALPix Nside
 one full-sky trace with the grids already resident in HBM.
 
 Printed: one JSON line with ray-steps/s of the mixed-precision kernel (the
-headline `value`), the plain FP64 kernel's rate beside it, the roofline of
-the dominant kernel, the per-map error of both kernels against the host FP64
+headline `value`, reading the bicubic cell records), the plain FP64 kernel's
+rate and the mixed kernel on the raw padded cubes beside it, the roofline of
+the dominant kernel, the per-map error of every kernel against the host FP64
 oracle on a seeded 1% pixel subset, and the oracle's own CPU rate.
 """
 from __future__ import annotations
@@ -28,15 +29,19 @@ import __graft_entry__  # noqa: E402
 # from rocprofv3 counters SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP64
 # (per wavefront instruction, x 64 lanes) of one launch of this workload,
 # divided by its 3,221,225,472 ray-steps (profiles/trace_kernel_512cube_nside1024_counters.md).
-FLOPS_PER_RAY_STEP = {"mixed": 12633.0, "fp64": 12929.0}
+FLOPS_PER_RAY_STEP = {"mixed_taps": 12633.0, "fp64": 12929.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
-HBM_BYTES_PER_LAUNCH = {"mixed": 16.42e9, "fp64": 25.42e9}
+HBM_BYTES_PER_LAUNCH = {"mixed_taps": 16.42e9, "fp64": 25.42e9}
 # The reference algorithm's count (oracle with an operation-counting scalar).
 REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
 # MI355X published vector peaks (FP32 counts packed FP32; this build disables
 # packed FP32, so 78.6 TFLOP/s is the attainable FP32 ceiling).
-PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3, "mixed_taps": 157.3, "ref": 157.3}
+# kernel name -> (precision, device layout it reads); None = the padded cubes
+# every context holds
+KERNELS = {"mixed": ("mixed", "bicubic"), "mixed_taps": ("mixed", "taps"), "fp64": ("fp64", None),
+           "ref": ("ref", None)}
 UNPACKED_FP32_TFLOPS = 78.6
 
 
@@ -49,16 +54,20 @@ def parse():
     p.add_argument("--nside", type=int, default=1024)
     p.add_argument("--n-steps", type=int, default=256)
     p.add_argument("--seed", type=int, default=2)
-    p.add_argument("--kernels", default="mixed,fp64")
+    p.add_argument("--kernels", default="mixed,fp64,mixed_taps",
+                   help="comma list of: mixed (FP32 interpolation from bicubic cell records), "
+                        "mixed_taps (same arithmetic on the padded cubes), fp64, ref")
+    p.add_argument("--brick", type=int, default=8, help="column brick edge of the bicubic records")
     p.add_argument("--order", default="nested", choices=["nested", "linear"])
     p.add_argument("--subset-frac", type=float, default=0.01)
     p.add_argument("--no-oracle", action="store_true", help="skip the CPU oracle leg (accuracy + cpu baseline)")
     p.add_argument("--cpu-threads", type=int, default=16)
-    p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch"],
+    p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch", "layouts"],
                    help="headline: 512^3 / Nside 1024 / 256 steps (the JSON line the driver reads); "
                         "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
                         "Nside 64 / 64 steps gradient with finite-difference check; batch: 48 uneven "
-                        "instances in one persistent launch")
+                        "instances in one persistent launch; layouts: the headline workload's mixed "
+                        "kernel on the padded cubes and on bicubic records of several brick edges")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -247,6 +256,37 @@ def workload_batch(args, pkg, fields, dev):
     plan.close()
 
 
+def workload_layouts(args, pkg, fields, dev):
+    """512^3 / Nside 1024 / 256 steps, mixed kernel: padded cubes vs bicubic records with column
+    bricks of edge 1 (row-major columns), 4, 8 and 16, each in NESTED and linear lane order."""
+    import torch
+    n, nside, n_steps = args.grid, args.nside, args.n_steps
+    rays = 12 * nside * nside
+    tr = pkg.Tracer(0)
+    phi, shift_il, shift = synth(fields, n, args.seed, dev)
+    del shift
+    tr.upload(phi, shift_il)
+    del phi, shift_il
+    torch.cuda.empty_cache()
+    out = torch.empty(6 * rays, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {}
+    for lay, brick in (("taps", 8), ("bicubic", 1), ("bicubic", 4), ("bicubic", 8), ("bicubic", 16)):
+        t0 = time.time()
+        tr.set_layout(pkg.LAYOUT_TAPS if lay == "taps" else pkg.LAYOUT_BICUBIC, brick)
+        torch.cuda.synchronize()
+        build_s = time.time() - t0
+        name = lay if lay == "taps" else f"bicubic_brick{brick}"
+        res[name] = {"build_s": build_s, "record_bytes": tr.layout()[2]}
+        for order_name, order in (("nested", pkg.ORDER_NESTED), ("linear", pkg.ORDER_LINEAR)):
+            tr.trace_device(nside, n_steps, out, precision=pkg.PREC_MIXED, order=order, stream=stream)
+            ms = tr.last_trace_ms()
+            res[name][order_name] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0)}
+            print(f"[bench] layouts {name} {order_name}: {ms:.1f} ms", file=sys.stderr, flush=True)
+    print(json.dumps({"workload": "layouts", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
+                                                        "rays": rays, "precision": "mixed"}, "results": res}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -269,7 +309,8 @@ def main():
         print("[bench] calibration copy: 1 GiB read + 1 GiB written", file=sys.stderr, flush=True)
         del src, dst
     if args.workload != "headline":
-        {"large": workload_large, "adjoint": workload_adjoint, "batch": workload_batch}[args.workload](
+        {"large": workload_large, "adjoint": workload_adjoint, "batch": workload_batch,
+         "layouts": workload_layouts}[args.workload](
             args, pkg, fields, dev)
         return
     n, nside, n_steps = args.grid, args.nside, args.n_steps
@@ -295,8 +336,18 @@ def main():
     rng = np.random.default_rng(8)
     subset = np.sort(rng.choice(rays, max(1, int(round(args.subset_frac * rays))), replace=False))
     subset_t = torch.from_numpy(subset).to(dev)
-    for name in [k for k in args.kernels.split(",") if k]:
-        prec = pkg.PRECISIONS[name]
+    names = [k for k in args.kernels.split(",") if k]
+    # kernels on the padded cubes first, so the bicubic records are built once
+    names.sort(key=lambda k: KERNELS[k][1] == "bicubic")
+    layout_s = {}
+    for name in names:
+        prec_name, layout = KERNELS[name]
+        prec = pkg.PRECISIONS[prec_name]
+        if layout is not None:
+            t0 = time.time()
+            tr.set_layout(pkg.LAYOUT_BICUBIC if layout == "bicubic" else pkg.LAYOUT_TAPS, args.brick)
+            torch.cuda.synchronize()
+            layout_s[layout] = time.time() - t0
         for _ in range(args.warmup):
             tr.trace_device(nside, n_steps, out, precision=prec, order=order, stream=stream)
         torch.cuda.synchronize()
@@ -376,6 +427,9 @@ def main():
         "kernels": results,
         "mixed_over_fp64": (results["mixed"]["ray_steps_per_s"] / results["fp64"]["ray_steps_per_s"]
                             if "mixed" in results and "fp64" in results else None),
+        "layout": {"mixed": f"bicubic cell records, {args.brick}x{args.brick} column bricks",
+                   "mixed_taps": "padded periodic cubes", "fp64": "padded periodic cubes (FP64 copies)",
+                   "build_s": layout_s},
         "map_error_vs_fp64_oracle": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

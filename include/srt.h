@@ -61,6 +61,22 @@ typedef enum {
     SRT_SHIFT_F16_SCALED = 1 /* binary16 of B * 2^27, the reference's storage (.cu:585-597, 241-244) */
 } srt_shift_format;
 
+/* Device layout of the snapshot (no counterpart in the reference, which
+ * keeps the potential as a linear float array and the shift as three planar
+ * binary16 arrays, .cu:738-855). */
+typedef enum {
+    /* Padded periodic cubes of grid values (4 (n+3)^3 floats); every form
+     * interpolates from them. */
+    SRT_LAYOUT_TAPS = 0,
+    /* In addition, per field, cell column and plane, the 16 monomial
+     * coefficients of the Catmull-Rom bicubic of the cell (64 bytes; the
+     * 4 x 4 x 4 stencil of a cell is 256 contiguous bytes), columns tiled in
+     * brick x brick bricks: 4 * 64 * n^2 * (n + 3) bytes (34.6 GB at 512^3).
+     * SRT_PREC_MIXED traces (single and batched) read these records; the
+     * other forms keep reading the cubes. */
+    SRT_LAYOUT_BICUBIC = 1
+} srt_layout;
+
 /* Order in which rays are assigned to GPU lanes (outputs always land in the
  * caller's slot order). */
 typedef enum {
@@ -94,6 +110,17 @@ const char* srt_last_error(void);
  * Replaces any previous snapshot of the context. */
 int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64_t nx, int64_t ny, int64_t nz,
                int inputs_on_device, int shift_format);
+
+/* Selects the device layout of this context's snapshot (srt_layout) and the
+ * column brick edge (a power of two in [1, 64]; 1 = row-major columns).
+ * Builds the layout's data now if a snapshot is uploaded, else at the next
+ * srt_upload; the setting persists across uploads.  Replaces the reference's
+ * fixed device formats (.cu:738-855).  Fails with SRT_ERR_OUT_OF_MEMORY
+ * (and falls back to SRT_LAYOUT_TAPS) if the records do not fit. */
+int srt_set_layout(srt_context* ctx, int layout, int brick);
+
+/* Current layout, brick edge and bytes of bicubic records held (0 if none). */
+int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_bytes);
 
 /* Traces rays and writes the six maps (raytracing_kernel, .cu:552-583, with
  * the pixel angles of .cu:857-892 computed on the device).

@@ -32,10 +32,14 @@ SHIFT_F16_SCALED = 1
 ORDER_LINEAR = 0
 ORDER_NESTED = 1
 
+LAYOUT_TAPS = 0
+LAYOUT_BICUBIC = 1
+
 MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "deflection1map", "deflection2map")
 
 # C-ABI functions declared in include/srt.h
-EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_trace",
+EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_set_layout",
+           "srt_get_layout", "srt_trace",
            "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_configure",
            "srt_plan_get_config", "srt_plan_autotune", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
            "srt_grid_info", "srt_version")
@@ -74,6 +78,8 @@ def lib():
         L.srt_last_error.restype = ctypes.c_char_p
         L.srt_version.restype = ctypes.c_char_p
         L.srt_upload.argtypes = [P, P, P, i64, i64, i64, i32, i32]
+        L.srt_set_layout.argtypes = [P, i32, i32]
+        L.srt_get_layout.argtypes = [P, P, P, P]
         L.srt_trace.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P, P]
         L.srt_trace_host.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P]
         L.srt_trace_adjoint.argtypes = [P, i64, i64, i64, P, i32, P, i32, P, P, P, P, P]
@@ -163,6 +169,22 @@ class Tracer:
         _check(lib().srt_upload(self._ctx, _ptr(phi), _ptr(shift_xyz), nx, ny, nz, on_dev, shift_format),
                "srt_upload")
         self.n = nx
+
+    def set_layout(self, layout: int, brick: int = 8) -> None:
+        """Device layout of the snapshot (LAYOUT_TAPS or LAYOUT_BICUBIC) and column brick edge.
+
+        LAYOUT_BICUBIC adds per-cell bicubic coefficient records, which mixed
+        traces read; they are built now if a snapshot is uploaded, else at the
+        next upload.
+        """
+        _check(lib().srt_set_layout(self._ctx, layout, brick), "srt_set_layout")
+
+    def layout(self) -> tuple[int, int, int]:
+        """(layout, brick edge, bytes of bicubic records held)."""
+        lay, brick, nbytes = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(lib().srt_get_layout(self._ctx, ctypes.byref(lay), ctypes.byref(brick), ctypes.byref(nbytes)),
+               "srt_get_layout")
+        return lay.value, brick.value, nbytes.value
 
     def trace_device(self, nside: int, n_steps: int, out, *, pix_begin: int = 0, count: int | None = None,
                      pix_list=None, observer=None, precision: int = PREC_FP64, order: int = ORDER_LINEAR,

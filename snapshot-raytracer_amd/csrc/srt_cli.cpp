@@ -18,8 +18,10 @@
 //
 // Environment (optional): SRT_PRECISION = mixed (default) | fp64 | ref selects
 // the kernel arithmetic; SRT_SHIFT = f32 (default) | f16 selects the device
-// shift storage (f16 is the reference's binary16 x 2^27); SRT_DEVICE = HIP
-// device ordinal.
+// shift storage (f16 is the reference's binary16 x 2^27); SRT_LAYOUT = auto
+// (default) | bicubic | taps selects the device layout the mixed kernel reads
+// (auto: bicubic cell records if they fit in device memory, else the padded
+// cubes); SRT_DEVICE = HIP device ordinal.
 #include <cerrno>
 #include <chrono>
 #include <cstdint>
@@ -95,6 +97,11 @@ int main(int argc, char* argv[]) {
         std::cerr << "Invalid SRT_SHIFT: " << shift_s << " (f32 or f16)" << std::endl;
         return -1;
     }
+    const std::string layout_s = env_or("SRT_LAYOUT", "auto");
+    if (layout_s != "auto" && layout_s != "bicubic" && layout_s != "taps") {
+        std::cerr << "Invalid SRT_LAYOUT: " << layout_s << " (auto, bicubic or taps)" << std::endl;
+        return -1;
+    }
     int64_t device = 0;
     if (!parse_int64(env_or("SRT_DEVICE", "0"), device) || device < 0) {
         std::cerr << "Invalid SRT_DEVICE" << std::endl;
@@ -148,6 +155,17 @@ int main(int argc, char* argv[]) {
     }
     std::vector<float>().swap(phi);
     std::vector<float>().swap(shift);
+    if (precision == SRT_PREC_MIXED && layout_s != "taps") {
+        rc = srt_set_layout(ctx, SRT_LAYOUT_BICUBIC, 8);
+        if (rc == SRT_ERR_OUT_OF_MEMORY && layout_s == "auto") {
+            std::cout << "Bicubic cell records do not fit in device memory; tracing from the padded grids."
+                      << std::endl;
+        } else if (rc != SRT_OK) {
+            std::cerr << "Device setup failed: " << srt_strerror(rc) << ": " << srt_last_error() << std::endl;
+            srt_destroy(ctx);
+            return -1;
+        }
+    }
 
     std::cout << "Launching raytracing kernel (" << prec_s << ")..." << std::endl;
     std::vector<float> maps((size_t)(6 * count));

@@ -149,7 +149,57 @@ struct Grids {
     // converts no taps); null when not built
     const double* phi64;
     const double* b64[3];
+    // bicubic cell records (potential, B_x, B_y, B_z), see record_offset;
+    // null when the context holds only the padded cubes
+    const float* rec[4];
+    int rec_shift;  // log2 of the column brick edge
+    int rec_nb;     // column bricks per row: ceil(n / brick)
 };
+
+// ------------------------------------------------------ bicubic records ----
+// Bicubic cell records.  For every cell column (i, j), i, j in [0, n), and
+// every padded plane p in [0, n + 3) (grid plane p - 1 mod n) a record holds
+// the 16 monomial coefficients of the Catmull-Rom bicubic of that cell on that
+// plane,
+//     f(tx, ty) = sum_{a,b} v(i - 1 + a, j - 1 + b) w_a(tx) w_b(ty)
+//               = sum_{p,q} c[p][q] tx^p ty^q,
+// stored as c[p][0..3] for p = 0..3 (64 bytes).  The records of one column
+// are consecutive in the plane index, so the 4 x 4 x 4 stencil of cell
+// (i, j, k) is the 256 contiguous bytes of planes k .. k + 3.  Columns are
+// tiled in bricks of 2^s x 2^s (s = rec_shift): column (i, j) is number
+//     ((j >> s) * nb + (i >> s)) * 4^s + (j & m) * 2^s + (i & m),  m = 2^s - 1,
+// so rays that cross neighbouring columns stay within a few pages.
+__device__ __forceinline__ size_t record_offset(const Grids& G, int i, int j, int k) {
+    const int s = G.rec_shift;
+    const int m = (1 << s) - 1;
+    const uint32_t col = ((uint32_t)((j >> s) * G.rec_nb + (i >> s)) << (2 * s)) + ((uint32_t)(j & m) << s) +
+                         (uint32_t)(i & m);
+    return ((size_t)col * (size_t)G.pad + (size_t)k) * 16;
+}
+
+// Catmull-Rom weights as polynomials: w_a(t) = sum_p kCrMonomial[a][p] t^p
+// (the polynomials of cr_axis below, by power of t).
+constexpr double kCrMonomial[4][4] = {{0.0, -0.5, 1.0, -0.5},
+                                      {1.0, 0.0, -2.5, 1.5},
+                                      {0.0, 0.5, 2.0, -1.5},
+                                      {0.0, 0.0, -0.5, 0.5}};
+
+// t, 2t, 3t and 3t^2: the factors of Horner evaluation of a cubic and of its
+// first and (halved) second derivative.
+template <class R>
+struct Powers {
+    R t, t2, t3, t3sq;
+};
+
+template <class R>
+__device__ __forceinline__ Powers<R> powers(R t) {
+    Powers<R> p;
+    p.t = t;
+    p.t2 = t + t;
+    p.t3 = R(3) * t;
+    p.t3sq = p.t3 * t;
+    return p;
+}
 
 // Wraps floor(x) into [0, n) and returns the offset x - floor(x) (exact in FP64).
 __device__ __forceinline__ int locate(double x, int n, double inv_n, double& t) {
@@ -250,6 +300,62 @@ __device__ __forceinline__ Interp<R> tricubic(const GT* __restrict__ base, size_
     return o;
 }
 
+// Tricubic value/gradient/Hessian from the four bicubic records of planes
+// k .. k + 3 at `rec`: per plane, the bicubic and its x, y, xx, xy, yy
+// derivatives by Horner's rule (38 multiply-adds), then the Catmull-Rom
+// z-pass over the four planes (40) and two doublings: 194 operations, against
+// 328 multiply-adds for tricubic() on the raw taps.  The same interpolant as tricubic(): only the
+// order of operations differs.
+template <class R>
+__device__ __forceinline__ Interp<R> bicubic_column(const R* __restrict__ rec, const Powers<R>& px,
+                                                    const Powers<R>& py, const Axis<R>& az) {
+    R P00[4], P10[4], P20h[4], P01[4], P11[4], P02h[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        R hv[4], hd[4], hh[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const auto c = load_row(rec + 16 * r + 4 * p);
+            hv[p] = c.x + py.t * (c.y + py.t * (c.z + py.t * c.w));  // sum_q c[p][q] ty^q
+            hd[p] = c.y + py.t2 * c.z + py.t3sq * c.w;                 // d/dty
+            hh[p] = c.z + py.t3 * c.w;                                 // (d2/dty2) / 2
+        }
+        P00[r] = hv[0] + px.t * (hv[1] + px.t * (hv[2] + px.t * hv[3]));
+        P10[r] = hv[1] + px.t2 * hv[2] + px.t3sq * hv[3];
+        P20h[r] = hv[2] + px.t3 * hv[3];
+        P01[r] = hd[0] + px.t * (hd[1] + px.t * (hd[2] + px.t * hd[3]));
+        P11[r] = hd[1] + px.t2 * hd[2] + px.t3sq * hd[3];
+        P02h[r] = hh[0] + px.t * (hh[1] + px.t * (hh[2] + px.t * hh[3]));
+    }
+    Interp<R> o;
+    o.v = az.w[0] * P00[0];
+    o.gz = az.d[0] * P00[0];
+    o.hzz = az.dd[0] * P00[0];
+    o.gx = az.w[0] * P10[0];
+    o.hxz = az.d[0] * P10[0];
+    o.gy = az.w[0] * P01[0];
+    o.hyz = az.d[0] * P01[0];
+    o.hxx = az.w[0] * P20h[0];
+    o.hxy = az.w[0] * P11[0];
+    o.hyy = az.w[0] * P02h[0];
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+        o.v += az.w[r] * P00[r];
+        o.gz += az.d[r] * P00[r];
+        o.hzz += az.dd[r] * P00[r];
+        o.gx += az.w[r] * P10[r];
+        o.hxz += az.d[r] * P10[r];
+        o.gy += az.w[r] * P01[r];
+        o.hyz += az.d[r] * P01[r];
+        o.hxx += az.w[r] * P20h[r];
+        o.hxy += az.w[r] * P11[r];
+        o.hyy += az.w[r] * P02h[r];
+    }
+    o.hxx += o.hxx;
+    o.hyy += o.hyy;
+    return o;
+}
+
 // ------------------------------------------------------------------ RHS ----
 // Precision policy of one kernel form.
 //   RI: interpolation arithmetic and the type the shift-coupling
@@ -257,17 +363,23 @@ __device__ __forceinline__ Interp<R> tricubic(const GT* __restrict__ base, size_
 //       derivative type; kRefOffset: cell offset taken as float(x) - float(i)
 //       (.cu:228-230) instead of the exact x - i.
 //   kStateInLds: the kernel keeps the RK4 state and stage sum in LDS;
-//   kGrid64: interpolate from the FP64 copies of the grids.
-template <class RI_, class RA_, bool kRefOffset_, bool kStateInLds_ = false, bool kGrid64_ = false>
+//   kGrid64: interpolate from the FP64 copies of the grids;
+//   kBicubic: interpolate from the bicubic cell records (FP32 forms);
+//   kWavesPerSimd: wavefronts per SIMD the register allocation must allow.
+template <class RI_, class RA_, bool kRefOffset_, bool kStateInLds_ = false, bool kGrid64_ = false,
+          bool kBicubic_ = false, int kWavesPerSimd_ = 1>
 struct Form {
     using RI = RI_;
     using RA = RA_;
     static constexpr bool kRefOffset = kRefOffset_;
     static constexpr bool kStateInLds = kStateInLds_;
     static constexpr bool kGrid64 = kGrid64_;
+    static constexpr bool kBicubic = kBicubic_;
+    static constexpr int kWavesPerSimd = kWavesPerSimd_;
 };
 using FormFP64 = Form<double, double, false, true, true>;
 using FormMixed = Form<float, float, false, false, false>;
+using FormMixedBicubic = Form<float, float, false, true, false, true, 2>;
 using FormRef = Form<float, double, true, false, false>;
 
 template <class F>
@@ -292,6 +404,16 @@ __device__ __forceinline__ R rnorm3(R a, R b, R c) {
 template <class R>
 __device__ __forceinline__ R sqrt_r(R a) {
     return sqrt(a);
+}
+
+// 1 / a: the hardware reciprocal (1 ulp) in FP32, IEEE division in FP64
+template <class R>
+__device__ __forceinline__ R rcp_r(R a) {
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_amdgcn_rcpf(a);
+    } else {
+        return R(1) / a;
+    }
 }
 
 // Stage state in FP64: position x[3], direction nd[3] (unnormalised), D, D'.
@@ -346,21 +468,40 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
             t1[a] = cell_offset<F>(xs, ts);
         }
     }
-    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
-
-    // Catmull-Rom weights of the unstaggered offsets, shared by all four
-    // interpolations; each shift component adds the set of its own staggered axis
-    const Axis<RI> w0[3] = {cr_axis<RI>(t0[0]), cr_axis<RI>(t0[1]), cr_axis<RI>(t0[2])};
+    // Field f (0 potential, 1 + a shift component a) at its cell: the shift
+    // component a uses the staggered cell along axis a.  Raw taps: Catmull-Rom
+    // weights of the unstaggered offsets are shared by all four fields, each
+    // shift component adds the set of its own staggered axis.  Bicubic
+    // records: Horner factors of the x and y offsets, Catmull-Rom weights of
+    // the z offset.
+    auto interp = [&](int f) -> Interp<RI> {
+        const int a = f - 1;
+        const int ix = (a == 0) ? i1[0] : i0[0];
+        const int iy = (a == 1) ? i1[1] : i0[1];
+        const int iz = (a == 2) ? i1[2] : i0[2];
+        if constexpr (F::kBicubic) {
+            static_assert(sizeof(RI) == 4, "bicubic records are FP32");
+            const Powers<RI> px = powers<RI>(a == 0 ? t1[0] : t0[0]);
+            const Powers<RI> py = powers<RI>(a == 1 ? t1[1] : t0[1]);
+            const Axis<RI> az = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
+            return bicubic_column<RI>(G.rec[f] + record_offset(G, ix, iy, iz), px, py, az);
+        } else {
+            const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
+            const size_t off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
+            const Axis<RI> wx = cr_axis<RI>(a == 0 ? t1[0] : t0[0]);
+            const Axis<RI> wy = cr_axis<RI>(a == 1 ? t1[1] : t0[1]);
+            const Axis<RI> wz = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
+            if constexpr (F::kGrid64)
+                return tricubic<RI>((f == 0 ? G.phi64 : G.b64[a]) + off, py, pz, wx, wy, wz);
+            else
+                return tricubic<RI>((f == 0 ? G.phi : G.b[a]) + off, py, pz, wx, wy, wz);
+        }
+    };
 
     // potential at x (.cu:462)
     RA phi, g[3], H[3][3];
     {
-        const size_t off = (size_t)i0[0] + (size_t)i0[1] * py + (size_t)i0[2] * pz;
-        Interp<RI> r;
-        if constexpr (F::kGrid64)
-            r = tricubic<RI>(G.phi64 + off, py, pz, w0[0], w0[1], w0[2]);
-        else
-            r = tricubic<RI>(G.phi + off, py, pz, w0[0], w0[1], w0[2]);
+        const Interp<RI> r = interp(0);
         phi = (RA)r.v;
         g[0] = (RA)r.gx;
         g[1] = (RA)r.gy;
@@ -381,16 +522,7 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
     RA Bv[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int ix = (a == 0) ? i1[0] : i0[0];
-        const int iy = (a == 1) ? i1[1] : i0[1];
-        const int iz = (a == 2) ? i1[2] : i0[2];
-        const Axis<RI> ws = cr_axis<RI>(t1[a]);
-        const size_t off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
-        Interp<RI> r;
-        if constexpr (F::kGrid64)
-            r = tricubic<RI>(G.b64[a] + off, py, pz, a == 0 ? ws : w0[0], a == 1 ? ws : w0[1], a == 2 ? ws : w0[2]);
-        else
-            r = tricubic<RI>(G.b[a] + off, py, pz, a == 0 ? ws : w0[0], a == 1 ? ws : w0[1], a == 2 ? ws : w0[2]);
+        const Interp<RI> r = interp(1 + a);
         Bv[a] = (RA)r.v;
         const RA gb[3] = {(RA)r.gx, (RA)r.gy, (RA)r.gz};
         const RA Hb[3][3] = {{(RA)r.hxx, (RA)r.hxy, (RA)r.hxz},
@@ -480,11 +612,16 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
     dy[15] = RA(4) * s2 * gam + S2H * D2;
     if (Dd > 0.5) {
         const RA q = sqrt_r<RA>(RA(4) + eps1 * eps1 + eps2 * eps2);
-        const RA inv_D2 = RA(1) / D2;
-        d13 += (s1 * s1 + s2 * s2) * inv_D2 / D;
+        const RA inv_D2 = rcp_r<RA>(D2);
+        if constexpr (sizeof(RA) == 4) {
+            d13 += (s1 * s1 + s2 * s2) * inv_D2 * rcp_r<RA>(D);
+            dy[18] = -(s1 * eps2 - s2 * eps1) * inv_D2 * rcp_r<RA>(RA(2) + q);
+        } else {
+            d13 += (s1 * s1 + s2 * s2) * inv_D2 / D;
+            dy[18] = -(s1 * eps2 - s2 * eps1) * inv_D2 / (RA(2) + q);
+        }
         dy[16] = RA(-2) * s1 * q * inv_D2;
         dy[17] = RA(-2) * s2 * q * inv_D2;
-        dy[18] = -(s1 * eps2 - s2 * eps1) * inv_D2 / (RA(2) + q);
     } else {
         dy[16] = RA(0);
         dy[17] = RA(0);

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -99,6 +100,54 @@ __global__ __launch_bounds__(256) void widen_kernel(const float* __restrict__ sr
         dst[q] = (double)src[q];
 }
 
+// Bicubic cell records of one field from its padded cube (layout: see
+// srt::record_offset).  Thread q = i + n (j + n p) builds the record of column
+// (i, j), padded plane p: the 4 x 4 taps v(i - 1 + a, j - 1 + b) of that plane
+// are padded entries (i + a, j + b, p); the x and y basis changes
+//     c[pw][qw] = sum_{a,b} M[a][pw] M[b][qw] v_ab,  M = srt::kCrMonomial,
+// run in FP64 and each coefficient is rounded to FP32 once.
+__global__ __launch_bounds__(256) void bicubic_records_kernel(const float* __restrict__ padded, int n, int P,
+                                                              int shift, int nb, float* __restrict__ rec) {
+    const int64_t total = (int64_t)n * n * P;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % n);
+        const int j = (int)((q / n) % n);
+        const int p = (int)(q / ((int64_t)n * n));
+        double X[4][4];  // X[b][pw] = sum_a M[a][pw] v_ab
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float* row = padded + (size_t)i + (size_t)P * ((size_t)(j + b) + (size_t)P * (size_t)p);
+            double v[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) v[a] = (double)row[a];
+#pragma unroll
+            for (int pw = 0; pw < 4; ++pw) {
+                double c = 0.0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) c += srt::kCrMonomial[a][pw] * v[a];
+                X[b][pw] = c;
+            }
+        }
+        srt::Grids G;
+        G.pad = P;
+        G.rec_shift = shift;
+        G.rec_nb = nb;
+        float* dst = rec + srt::record_offset(G, i, j, p);
+#pragma unroll
+        for (int pw = 0; pw < 4; ++pw) {
+            float c4[4];
+#pragma unroll
+            for (int qw = 0; qw < 4; ++qw) {
+                double c = 0.0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) c += srt::kCrMonomial[b][qw] * X[b][pw];
+                c4[qw] = (float)c;
+            }
+            __builtin_memcpy(dst + 4 * pw, c4, sizeof(c4));
+        }
+    }
+}
+
 // ---------------------------------------------------------------- trace ----
 struct TraceArgs {
     srt::Grids G;
@@ -111,7 +160,19 @@ struct TraceArgs {
     double obs[3];
     float* out;
     double* state;
+    int xcd_groups;  // see tile_of_block
 };
+
+// Ray tile traced by workgroup `b` of `nb`.  The dispatcher hands workgroups
+// to the chip's XCDs round-robin (b -> XCD b mod 8 on MI355X), and each XCD
+// has its own L2.  With g = 8 groups, the workgroups an XCD receives trace
+// one contiguous range of tiles, so rays of neighbouring sky patches (which
+// read neighbouring cells) share an L2; g = 1 is the identity.
+__device__ __forceinline__ int64_t tile_of_block(int64_t b, int64_t nb, int g) {
+    if (g <= 1) return b;
+    const int64_t x = b % g, idx = b / g, per = nb / g, extra = nb % g;
+    return x * per + (x < extra ? x : extra) + idx;
+}
 
 // Traces one ray (RING pixel `pix`) and writes its maps to slot `slot` of
 // the map-major output (and the final state if `state` is not null).
@@ -256,8 +317,8 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
 }
 
 template <class F>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TraceArgs A) {
-    const int64_t lane_ray = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void trace_kernel(TraceArgs A) {
+    const int64_t lane_ray = tile_of_block(blockIdx.x, gridDim.x, A.xcd_groups) * kBlock + threadIdx.x;
     if (lane_ray >= A.count) return;
     int64_t pix, slot;
     if (A.order == SRT_ORDER_NESTED) {
@@ -548,6 +609,14 @@ struct srt_context {
     size_t adj_bytes = 0;
     // FP64 copies of the padded grids, built on the first FP64 trace
     double* grids64 = nullptr;
+    // bicubic cell records of the four fields (SRT_LAYOUT_BICUBIC)
+    int layout = SRT_LAYOUT_TAPS;
+    int brick_shift = 3;
+    float* rec = nullptr;
+    size_t rec_field_floats = 0;
+    // workgroup -> tile grouping of trace launches (tile_of_block); from
+    // SRT_XCD_GROUPS at context creation, default 8 (the XCDs of an MI355X)
+    int xcd_groups = 8;
 };
 
 namespace {
@@ -567,6 +636,52 @@ void set_grids64(const srt_context* ctx, srt::Grids& G) {
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
     G.phi64 = ctx->grids64;
     for (int a = 0; a < 3; ++a) G.b64[a] = ctx->grids64 ? ctx->grids64 + (size_t)(1 + a) * pvox : nullptr;
+}
+
+int brick_columns(int n, int shift) {
+    return (n + (1 << shift) - 1) >> shift;
+}
+
+void free_records(srt_context* ctx) {
+    if (ctx->rec) (void)hipFree(ctx->rec);
+    ctx->rec = nullptr;
+    ctx->rec_field_floats = 0;
+}
+
+// Builds the bicubic cell records of the four fields from the padded cubes.
+int build_records(srt_context* ctx) {
+    free_records(ctx);
+    const int n = ctx->n, P = ctx->pad, s = ctx->brick_shift, nb = brick_columns(n, s);
+    const size_t cols = (size_t)nb * nb << (2 * s);
+    const size_t field = cols * (size_t)P * 16;
+    const hipError_t ea = hipMalloc(&ctx->rec, 4 * field * sizeof(float));
+    if (ea != hipSuccess) {
+        ctx->rec = nullptr;
+        (void)hipGetLastError();  // a failed allocation must not surface at the next launch check
+        return fail(ea == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
+                    "bicubic records: hipMalloc of " + std::to_string(4 * field * sizeof(float)) + " bytes: " +
+                        hipGetErrorName(ea));
+    }
+    ctx->rec_field_floats = field;
+    const size_t pvox = (size_t)P * P * P;
+    const int64_t total = (int64_t)n * n * P;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    for (int f = 0; f < 4; ++f)
+        hipLaunchKernelGGL(bicubic_records_kernel, dim3(blocks), dim3(256), 0, 0, ctx->grids + (size_t)f * pvox, n, P,
+                           s, nb, ctx->rec + (size_t)f * field);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_records(ctx);
+        return fail(SRT_ERR_DEVICE, std::string("bicubic records: ") + hipGetErrorName(e));
+    }
+    return SRT_OK;
+}
+
+void set_records(const srt_context* ctx, srt::Grids& G) {
+    for (int f = 0; f < 4; ++f) G.rec[f] = ctx->rec ? ctx->rec + (size_t)f * ctx->rec_field_floats : nullptr;
+    G.rec_shift = ctx->brick_shift;
+    G.rec_nb = brick_columns(ctx->n, ctx->brick_shift);
 }
 
 }  // namespace
@@ -601,6 +716,10 @@ int srt_create(int device, srt_context** ctx) {
     SRT_HIP(hipSetDevice(device));
     srt_context* c = new srt_context();
     c->device = device;
+    if (const char* g = std::getenv("SRT_XCD_GROUPS")) {
+        const int v = std::atoi(g);
+        if (v >= 1 && v <= 64) c->xcd_groups = v;
+    }
     hipError_t e1 = hipEventCreate(&c->ev_start);
     hipError_t e2 = hipEventCreate(&c->ev_stop);
     if (e1 != hipSuccess || e2 != hipSuccess) {
@@ -617,6 +736,7 @@ int srt_destroy(srt_context* ctx) {
     if (ctx->grids) (void)hipFree(ctx->grids);
     if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
     if (ctx->grids64) (void)hipFree(ctx->grids64);
+    free_records(ctx);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     delete ctx;
@@ -654,6 +774,7 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
         SRT_HIP(hipFree(ctx->grids64));
         ctx->grids64 = nullptr;
     }
+    free_records(ctx);
     float* grids = nullptr;
     SRT_HIP(hipMalloc(&grids, 4 * pvox * sizeof(float)));
     const float* dphi = phi;
@@ -692,6 +813,40 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
     ctx->grids = grids;
     ctx->n = n;
     ctx->pad = P;
+    if (ctx->layout == SRT_LAYOUT_BICUBIC) return build_records(ctx);
+    return SRT_OK;
+}
+
+int srt_set_layout(srt_context* ctx, int layout, int brick) {
+    if (!ctx) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_set_layout: null context");
+    if (layout != SRT_LAYOUT_TAPS && layout != SRT_LAYOUT_BICUBIC)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_set_layout: unknown layout");
+    int shift = -1;
+    for (int s = 0; s <= 6; ++s)
+        if (brick == (1 << s)) shift = s;
+    if (shift < 0) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_set_layout: brick must be a power of two in [1, 64]");
+    SRT_HIP(hipSetDevice(ctx->device));
+    SRT_HIP(hipDeviceSynchronize());
+    const bool rebuild = layout == SRT_LAYOUT_BICUBIC && (!ctx->rec || shift != ctx->brick_shift);
+    ctx->layout = layout;
+    ctx->brick_shift = shift;
+    if (layout == SRT_LAYOUT_TAPS) {
+        free_records(ctx);
+        return SRT_OK;
+    }
+    if (ctx->grids && rebuild) {
+        const int rc = build_records(ctx);
+        if (rc != SRT_OK) ctx->layout = SRT_LAYOUT_TAPS;
+        return rc;
+    }
+    return SRT_OK;
+}
+
+int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_bytes) {
+    if (!ctx) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_get_layout: null context");
+    if (layout) *layout = ctx->layout;
+    if (brick) *brick = 1 << ctx->brick_shift;
+    if (record_bytes) *record_bytes = (int64_t)(4 * ctx->rec_field_floats * sizeof(float));
     return SRT_OK;
 }
 
@@ -723,6 +878,7 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
         if (rc != SRT_OK) return rc;
     }
     set_grids64(ctx, A.G);
+    set_records(ctx, A.G);
     A.G.n = ctx->n;
     A.G.pad = ctx->pad;
     A.G.inv_n = 1.0 / (double)ctx->n;
@@ -735,6 +891,7 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     for (int i = 0; i < 3; ++i) A.obs[i] = observer ? observer[i] : 0.5 * (double)ctx->n;
     A.out = out;
     A.state = final_state;
+    A.xcd_groups = ctx->xcd_groups;
     SRT_HIP(hipEventRecord(ctx->ev_start, st));
     if (count > 0) {
         const int64_t blocks = (count + kBlock - 1) / kBlock;
@@ -744,8 +901,12 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
                                    lds_bytes<srt::FormFP64>(), st, A);
                 break;
             case SRT_PREC_MIXED:
-                hipLaunchKernelGGL(trace_kernel<srt::FormMixed>, dim3((unsigned)blocks), dim3(kBlock),
-                                   lds_bytes<srt::FormMixed>(), st, A);
+                if (ctx->rec)
+                    hipLaunchKernelGGL(trace_kernel<srt::FormMixedBicubic>, dim3((unsigned)blocks), dim3(kBlock),
+                                       lds_bytes<srt::FormMixedBicubic>(), st, A);
+                else
+                    hipLaunchKernelGGL(trace_kernel<srt::FormMixed>, dim3((unsigned)blocks), dim3(kBlock),
+                                       lds_bytes<srt::FormMixed>(), st, A);
                 break;
             case SRT_PREC_REF:
                 hipLaunchKernelGGL(trace_kernel<srt::FormRef>, dim3((unsigned)blocks), dim3(kBlock),
@@ -803,6 +964,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
         if (rc != SRT_OK) return rc;
     }
     set_grids64(ctx, A.G);
+    set_records(ctx, A.G);
     A.nside = nside;
     A.pix_begin = pix_begin;
     A.count = count;
@@ -844,6 +1006,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
 struct srt_plan {
     int device = 0;
     int precision = SRT_PREC_MIXED;
+    bool bicubic = false;  // mixed plan whose contexts all hold bicubic records
     int n_inst = 0;
     int64_t total_rays = 0, total_ray_steps = 0;
     std::vector<srt_batch_item> items;
@@ -873,6 +1036,8 @@ void plan_drop_graph(srt_plan* p) {
 int plan_build_table(srt_plan* p) {
     std::vector<BatchInst> h(p->n_inst);
     int64_t tiles = 0;
+    p->bicubic = p->precision == SRT_PREC_MIXED;
+    for (int i = 0; i < p->n_inst; ++i) p->bicubic = p->bicubic && p->items[i].ctx->rec != nullptr;
     for (int i = 0; i < p->n_inst; ++i) {
         const srt_batch_item& it = p->items[i];
         srt_context* c = it.ctx;
@@ -888,6 +1053,7 @@ int plan_build_table(srt_plan* p) {
             if (rc != SRT_OK) return rc;
         }
         set_grids64(c, b.G);
+        set_records(c, b.G);
         b.nside = it.nside;
         b.pix_begin = it.pix_begin;
         b.count = it.count;
@@ -912,9 +1078,14 @@ int plan_enqueue(srt_plan* p, hipStream_t st) {
                                st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->d_counter);
             break;
         case SRT_PREC_MIXED:
-            hipLaunchKernelGGL(batch_kernel<srt::FormMixed>, dim3(p->blocks), dim3(kBlock),
-                               lds_bytes<srt::FormMixed>(), st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays,
-                               p->d_counter);
+            if (p->bicubic)
+                hipLaunchKernelGGL(batch_kernel<srt::FormMixedBicubic>, dim3(p->blocks), dim3(kBlock),
+                                   lds_bytes<srt::FormMixedBicubic>(), st, p->d_inst, p->n_inst, p->total_tiles,
+                                   p->tile_rays, p->d_counter);
+            else
+                hipLaunchKernelGGL(batch_kernel<srt::FormMixed>, dim3(p->blocks), dim3(kBlock),
+                                   lds_bytes<srt::FormMixed>(), st, p->d_inst, p->n_inst, p->total_tiles,
+                                   p->tile_rays, p->d_counter);
             break;
         default:
             hipLaunchKernelGGL(batch_kernel<srt::FormRef>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormRef>(),

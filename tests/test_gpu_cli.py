@@ -35,15 +35,17 @@ def run(d, args, env=None):
                           env=e)
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_cli_full_sky_matches_library(srt, files, precision):
+@pytest.mark.parametrize("precision,layout", [("mixed", "auto"), ("mixed", "taps"), ("fp64", "auto")])
+def test_cli_full_sky_matches_library(srt, files, precision, layout):
     from snapshot_raytracer_amd import snapshot_io
     d, phi, shift, shift_il = files
     nside, steps = 4, 10
-    r = run(d, [str(nside), str(steps)], {"SRT_PRECISION": precision})
+    r = run(d, [str(nside), str(steps)], {"SRT_PRECISION": precision, "SRT_LAYOUT": layout})
     assert r.returncode == 0, r.stderr
     assert "All done!" in r.stdout
     tr = srt.Tracer(0)
+    if precision == "mixed" and layout == "auto":  # a 32^3 snapshot's records fit: the CLI reads them
+        tr.set_layout(srt.LAYOUT_BICUBIC, 8)
     tr.upload(phi, shift_il)
     ref = tr.trace(nside, steps, precision=srt.PRECISIONS[precision])
     tr.close()
@@ -72,4 +74,6 @@ def test_cli_reference_storage_mode(files):
     r = run(d, ["4", "6"], {"SRT_PRECISION": "ref", "SRT_SHIFT": "f16"})
     assert r.returncode == 0, r.stderr
     r = run(d, ["4", "6"], {"SRT_PRECISION": "bogus"})
+    assert r.returncode == 255
+    r = run(d, ["4", "6"], {"SRT_LAYOUT": "bogus"})
     assert r.returncode == 255
