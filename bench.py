@@ -29,10 +29,10 @@ import __graft_entry__  # noqa: E402
 # from rocprofv3 counters SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP64
 # (per wavefront instruction, x 64 lanes) of one launch of this workload,
 # divided by its 3,221,225,472 ray-steps (profiles/trace_kernel_512cube_nside1024_counters.md).
-FLOPS_PER_RAY_STEP = {"mixed_taps": 12633.0, "fp64": 12929.0}
+FLOPS_PER_RAY_STEP = {"mixed": 8662.0, "mixed_taps": 12485.0, "fp64": 12929.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
-HBM_BYTES_PER_LAUNCH = {"mixed_taps": 16.42e9, "fp64": 25.42e9}
+HBM_BYTES_PER_LAUNCH = {"mixed": 29.69e9, "mixed_taps": 4.88e9, "fp64": 7.99e9}
 # The reference algorithm's count (oracle with an operation-counting scalar).
 REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
 # MI355X published vector peaks (FP32 counts packed FP32; this build disables
@@ -96,7 +96,9 @@ def synth(fields, n, seed, dev):
 
 
 def workload_large(args, pkg, fields, dev):
-    """1024^3 potential + 3 x 1024^3 shift, Nside 2048 full sky, 512 steps, mixed, one launch."""
+    """1024^3 potential + 3 x 1024^3 shift, Nside 2048 full sky, 512 steps, mixed, one launch per
+    layout and lane order: bicubic cell records in 8x8 column bricks and in row-major columns (when
+    they fit in device memory), then the padded grids."""
     import torch
     n, nside, n_steps = 1024, 2048, 512
     rays = 12 * nside * nside
@@ -107,16 +109,33 @@ def workload_large(args, pkg, fields, dev):
     tr.upload(phi, shift_il)
     del phi, shift_il
     torch.cuda.empty_cache()
-    print(f"[bench] large: snapshot + upload {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    free, total = torch.cuda.mem_get_info()
+    print(f"[bench] large: snapshot + upload {time.time() - t0:.1f} s; device memory free {free / 1e9:.1f} of "
+          f"{total / 1e9:.1f} GB", file=sys.stderr, flush=True)
     out = torch.empty(6 * rays, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    res = {}
-    for order_name, order in (("nested", pkg.ORDER_NESTED), ("linear", pkg.ORDER_LINEAR)):
-        tr.trace_device(nside, n_steps, out, precision=pkg.PREC_MIXED, order=order, stream=stream)
-        ms = tr.last_trace_ms()
-        res[order_name] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0)}
-        print(f"[bench] large {order_name}: {ms:.0f} ms, {res[order_name]['ray_steps_per_s']:.3e} ray-steps/s",
-              file=sys.stderr, flush=True)
+    res = {"device_memory_free_gb": free / 1e9, "device_memory_total_gb": total / 1e9}
+
+    def run(name, orders):
+        res[name] = {"record_bytes": tr.layout()[2]}
+        for order_name, order in orders:
+            tr.trace_device(nside, n_steps, out, precision=pkg.PREC_MIXED, order=order, stream=stream)
+            ms = tr.last_trace_ms()
+            res[name][order_name] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0)}
+            print(f"[bench] large {name} {order_name}: {ms:.0f} ms, "
+                  f"{res[name][order_name]['ray_steps_per_s']:.3e} ray-steps/s", file=sys.stderr, flush=True)
+
+    both = (("nested", pkg.ORDER_NESTED), ("linear", pkg.ORDER_LINEAR))
+    try:
+        tr.set_layout(pkg.LAYOUT_BICUBIC, 8)
+        run("bicubic_brick8", both)
+        tr.set_layout(pkg.LAYOUT_BICUBIC, 1)
+        run("bicubic_brick1", (("nested", pkg.ORDER_NESTED),))
+    except pkg.SrtError as e:
+        res["bicubic"] = f"not built: {e}"
+        print(f"[bench] large: bicubic records not built ({e})", file=sys.stderr, flush=True)
+    tr.set_layout(pkg.LAYOUT_TAPS)
+    run("taps", both)
     print(json.dumps({"workload": "large", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
                                                       "rays": rays, "precision": "mixed"}, "results": res}))
 
@@ -195,48 +214,73 @@ def workload_adjoint(args, pkg, fields, dev):
 
 def workload_batch(args, pkg, fields, dev):
     """48 instances, grids from {128,192,256,384}^3, observers uniform in the box, n_steps in
-    [64, N/2], Nside from {128,256,512}, one random RING batch each (rng 7, seeds 1000 + 4 i)."""
+    [64, N/2], Nside from {128,256,512}, one random RING batch each (rng 7, seeds 1000 + 4 i).
+    Mixed precision; bicubic cell records for every instance when they all fit in device memory."""
     import numpy as np
     import torch
     rng = np.random.default_rng(7)
-    items = []
-    t0 = time.time()
+    params = []
     for i in range(48):
         n = int(rng.choice([128, 192, 256, 384]))
         nside = int(rng.choice([128, 256, 512]))
         b = int(rng.integers(0, 12))
         n_steps = int(rng.integers(64, n // 2 + 1))
         obs = rng.uniform(0, n, 3)
+        params.append((n, nside, b, n_steps, obs))
+    rec_bytes = sum(4 * 64 * n * n * (n + 3) for n, *_ in params)
+    taps_bytes = sum(4 * 4 * (n + 3) ** 3 for n, *_ in params)
+    free, total = torch.cuda.mem_get_info()
+    # room for the records next to the grids, the outputs and the generator's buffers
+    layout = "bicubic" if rec_bytes + taps_bytes + 8e9 < free else "taps"
+    print(f"[bench] batch: records {rec_bytes / 1e9:.1f} GB + grids {taps_bytes / 1e9:.1f} GB, free "
+          f"{free / 1e9:.1f} GB -> {layout}", file=sys.stderr, flush=True)
+    items = []
+    t0 = time.time()
+    for i, (n, nside, b, n_steps, obs) in enumerate(params):
         tr = pkg.Tracer(0)
+        if layout == "bicubic":
+            tr.set_layout(pkg.LAYOUT_BICUBIC, 8)
         phi, shift_il, _ = synth(fields, n, 1000 + 4 * i, dev)
         tr.upload(phi, shift_il)
         del phi, shift_il
+        torch.cuda.empty_cache()
         lo, cnt = pkg.batch_range(nside, b)
         items.append(dict(tracer=tr, nside=nside, pix_begin=lo, count=cnt, n_steps=n_steps, observer=obs,
                           out=torch.empty(6 * cnt, dtype=torch.float32, device=dev)))
     torch.cuda.synchronize()
     print(f"[bench] batch: 48 snapshots in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
-    total = sum(it["count"] * it["n_steps"] for it in items)
+    total_rs = sum(it["count"] * it["n_steps"] for it in items)
     stream = torch.cuda.Stream()
-    # one launch per instance, back to back
-    with torch.cuda.stream(stream):
-        for it in items:
-            it["tracer"].trace_device(it["nside"], it["n_steps"], it["out"], pix_begin=it["pix_begin"],
-                                      count=it["count"], observer=it["observer"], precision=pkg.PREC_MIXED,
-                                      stream=stream)
+
+    def sequential(order):
+        with torch.cuda.stream(stream):
+            for it in items:
+                it["tracer"].trace_device(it["nside"], it["n_steps"], it["out"], pix_begin=it["pix_begin"],
+                                          count=it["count"], observer=it["observer"], precision=pkg.PREC_MIXED,
+                                          order=order, stream=stream)
         stream.synchronize()
+
+    seq = {}
+    for name, order in (("linear", pkg.ORDER_LINEAR), ("nested", pkg.ORDER_NESTED)):
+        sequential(order)
         t0 = time.perf_counter()
-        for it in items:
-            it["tracer"].trace_device(it["nside"], it["n_steps"], it["out"], pix_begin=it["pix_begin"],
-                                      count=it["count"], observer=it["observer"], precision=pkg.PREC_MIXED,
-                                      stream=stream)
-        stream.synchronize()
-    seq_s = time.perf_counter() - t0
+        sequential(order)
+        wall = time.perf_counter() - t0
+        seq[name] = {"ms": wall * 1000, "ray_steps_per_s": total_rs / wall}
+        print(f"[bench] batch: 48 launches, {name} lane order: {wall * 1000:.1f} ms", file=sys.stderr, flush=True)
     plan = pkg.Plan(items, precision=pkg.PREC_MIXED)
     default_cfg = plan.config()[:2]
-    plan.launch(stream=stream)
-    stream.synchronize()
-    default_ms = plan.last_ms()
+    variants = {}
+    for name, order, groups in (("linear_1group", pkg.ORDER_LINEAR, 1), ("nested_1group", pkg.ORDER_NESTED, 1),
+                                ("nested_8groups", pkg.ORDER_NESTED, 8)):
+        plan.set_order(order, groups)
+        plan.launch(stream=stream)
+        stream.synchronize()
+        plan.launch(stream=stream)
+        stream.synchronize()
+        variants[name] = plan.last_ms()
+        print(f"[bench] batch plan {name}: {variants[name]:.1f} ms", file=sys.stderr, flush=True)
+    plan.set_order(pkg.ORDER_NESTED, 8)
     tile, blocks = plan.autotune(stream=stream)
     res = {}
     for graph in (False, True):
@@ -247,11 +291,14 @@ def workload_batch(args, pkg, fields, dev):
             plan.launch(stream=stream, graph=graph)
         stream.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        res["graph" if graph else "direct"] = {"ms": wall * 1000, "ray_steps_per_s": total / wall,
+        res["graph" if graph else "direct"] = {"ms": wall * 1000, "ray_steps_per_s": total_rs / wall,
                                                "kernel_ms": plan.last_ms()}
-    print(json.dumps({"workload": "batch", "instances": 48, "ray_steps": total,
-                      "sequential_launches": {"ms": seq_s * 1000, "ray_steps_per_s": total / seq_s},
-                      "plan_default": {"tile_rays": default_cfg[0], "blocks": default_cfg[1], "ms": default_ms},
+        print(f"[bench] batch plan autotuned ({tile}, {blocks}) graph={graph}: {wall * 1000:.1f} ms",
+              file=sys.stderr, flush=True)
+    print(json.dumps({"workload": "batch", "instances": 48, "ray_steps": total_rs, "layout": layout,
+                      "sequential_launches": seq,
+                      "plan_default_config": {"tile_rays": default_cfg[0], "blocks": default_cfg[1]},
+                      "plan_variants_ms": variants,
                       "autotuned": {"tile_rays": tile, "blocks": blocks}, "plan": res}))
     plan.close()
 
@@ -407,7 +454,8 @@ def main():
                     "hbm_frac": (HBM_BYTES_PER_LAUNCH[head] / (r["kernel_ms_avg"] / 1000.0) / 8e12
                                  if head in HBM_BYTES_PER_LAUNCH else None),
                     "flops_per_ray_step": flops, "reference_algorithm_flops_per_ray_step":
-                        REF_ALGO_FLOPS_PER_RAY_STEP, "kernel": f"trace_kernel<{head}>"}
+                        REF_ALGO_FLOPS_PER_RAY_STEP,
+                    "kernel": "trace_kernel<mixed, bicubic records>" if head == "mixed" else f"trace_kernel<{head}>"}
     line = {
         "metric": "ray-steps/s at 512^3, Nside 1024, 256 steps (mixed-precision kernel)",
         "value": r["ray_steps_per_s"],

@@ -81,8 +81,11 @@ typedef enum {
  * caller's slot order). */
 typedef enum {
     SRT_ORDER_LINEAR = 0, /* lane i traces slot i */
-    SRT_ORDER_NESTED = 1  /* full-sky traces only: lanes walk NESTED pixel order (compact sky
-                             patches per wavefront), results are written to RING slots */
+    SRT_ORDER_NESTED = 1  /* lanes walk the rays in NESTED order of their pixels (compact sky
+                             patches per wavefront; for a partial sky, a pixel list or a Nside
+                             that is not a power of two the rays are first sorted by base face
+                             and interleaved in-face coordinates on the device); results are
+                             written to the caller's slots, bit-identical to LINEAR */
 } srt_ray_order;
 
 /* Replaces the device selection and allocation bookkeeping of main()
@@ -135,8 +138,10 @@ int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_by
  *        eps1, eps2, rotation) in the precision of the kernel.
  *   stream: hipStream_t to launch on (null = the default stream).  The call is
  *        asynchronous with respect to the host.
- *   ray_order: srt_ray_order.  SRT_ORDER_NESTED needs pix_list == null,
- *        pix_begin == 0 and count == 12 * nside^2. */
+ *   ray_order: srt_ray_order.  SRT_ORDER_NESTED other than over the full sky
+ *        at a power-of-two Nside keeps a lane -> slot map (4 bytes per ray)
+ *        and sort space in the context: overlapping such calls on one
+ *        context must share a stream. */
 int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
               int n_steps, const double* observer, int precision, int ray_order, float* out,
               double* final_state, void* stream);
@@ -192,6 +197,13 @@ typedef struct srt_plan srt_plan;
  * All contexts must be on one device and stay alive while the plan is used. */
 int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt_plan** plan);
 int srt_plan_destroy(srt_plan* plan);
+
+/* Lane order within each item (srt_ray_order; default SRT_ORDER_NESTED: each
+ * item's rays sorted into compact sky patches once, when set) and the number
+ * of contiguous tile ranges the launch splits the work into (1..64; default
+ * 8: a workgroup starts on the range of its XCD, so neighbouring tiles share
+ * an L2, then helps the other ranges).  Outputs do not depend on either. */
+int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups);
 
 /* Tile size in rays (multiple of 64) and number of persistent 256-lane blocks. */
 int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks);

@@ -40,7 +40,7 @@ MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "defl
 # C-ABI functions declared in include/srt.h
 EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_set_layout",
            "srt_get_layout", "srt_trace",
-           "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_configure",
+           "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_set_order", "srt_plan_configure",
            "srt_plan_get_config", "srt_plan_autotune", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
            "srt_grid_info", "srt_version")
 
@@ -86,6 +86,7 @@ def lib():
         L.srt_plan_create.argtypes = [P, i32, i32, ctypes.POINTER(P)]
         L.srt_plan_destroy.argtypes = [P]
         L.srt_plan_configure.argtypes = [P, i32, i32]
+        L.srt_plan_set_order.argtypes = [P, i32, i32]
         L.srt_plan_get_config.argtypes = [P, P, P, P]
         L.srt_plan_autotune.argtypes = [P, P, P, P]
         L.srt_plan_launch.argtypes = [P, i32, P]
@@ -292,6 +293,10 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+    def set_order(self, order: int = ORDER_NESTED, tile_groups: int = 8) -> None:
+        """Lane order within each item and the number of XCD tile ranges (outputs do not change)."""
+        _check(lib().srt_plan_set_order(self._plan, order, tile_groups), "srt_plan_set_order")
 
     def configure(self, tile_rays: int, blocks: int) -> None:
         _check(lib().srt_plan_configure(self._plan, tile_rays, blocks), "srt_plan_configure")

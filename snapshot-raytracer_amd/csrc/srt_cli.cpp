@@ -170,7 +170,9 @@ int main(int argc, char* argv[]) {
     std::cout << "Launching raytracing kernel (" << prec_s << ")..." << std::endl;
     std::vector<float> maps((size_t)(6 * count));
     const auto t0 = std::chrono::steady_clock::now();
-    const int order = (batch < 0) ? SRT_ORDER_NESTED : SRT_ORDER_LINEAR;
+    // compact sky patches per wavefront (a batch's rays are sorted on the device);
+    // the maps do not depend on the lane order
+    const int order = SRT_ORDER_NESTED;
     rc = srt_trace_host(ctx, nside, istart, count, nullptr, (int)n_steps, nullptr, precision, order, maps.data(),
                         nullptr);
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

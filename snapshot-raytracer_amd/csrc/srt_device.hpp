@@ -62,8 +62,9 @@ __device__ __forceinline__ void ring_pixel_direction(int64_t n, int64_t p, doubl
     }
 }
 
-// NESTED pixel -> RING pixel (face / in-face coordinates, Gorski et al. 2005).
-// Used to hand each wavefront a compact 8 x 8 patch of the sky.
+// NESTED pixel -> RING pixel (face / in-face coordinates, Gorski et al. 2005),
+// for power-of-two Nside (the NESTED scheme's domain).  Used to hand each
+// wavefront a compact 8 x 8 patch of the sky in full-sky traces.
 __device__ __forceinline__ uint32_t compact_even_bits(uint64_t v) {
     v &= 0x5555555555555555ull;
     v = (v | (v >> 1)) & 0x3333333333333333ull;
@@ -104,6 +105,65 @@ __device__ __forceinline__ int64_t nest_to_ring(int64_t n, int64_t p) {
     else if (jp < 1)
         jp += 4 * n;
     return before + jp - 1;
+}
+
+// RING pixel -> lane-order key face * 4^L + interleave(ix, iy), L = the bit
+// length of n - 1, from the pixel's base face and in-face coordinates
+// (Gorski et al. 2005).  The key is unique for every Nside; for power-of-two
+// Nside it is the NESTED pixel index.  Sorting rays by it gives each
+// wavefront a compact patch of sky.
+__device__ __forceinline__ uint64_t spread_bits(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+__device__ __forceinline__ int64_t ring_order_key(int64_t n, int64_t p) {
+    const int64_t ncap = 2 * n * (n - 1), npix = 12 * n * n;
+    int64_t iring, iphi, kshift, nr;
+    int face;
+    if (p < ncap) {  // north polar cap: ring iring has 4 iring pixels
+        iring = (1 + isqrt64(1 + 2 * p)) / 2;
+        iphi = p + 1 - 2 * iring * (iring - 1);
+        kshift = 0;
+        nr = iring;
+        face = (int)((iphi - 1) / nr);
+    } else if (p < npix - ncap) {  // equatorial belt
+        const int64_t q = p - ncap, t = q / (4 * n);
+        iring = t + n;
+        iphi = q - t * 4 * n + 1;
+        kshift = (iring + n) & 1;
+        nr = n;
+        const int64_t ire = t + 1, irm = 2 * n + 2 - ire;
+        const int64_t ifm = (iphi - ire / 2 + n - 1) / n, ifp = (iphi - irm / 2 + n - 1) / n;
+        if (ifp == ifm)
+            face = (int)(ifp == 4 ? 4 : ifp + 4);
+        else if (ifp < ifm)
+            face = (int)ifp;
+        else
+            face = (int)(ifm + 8);
+    } else {  // south polar cap
+        const int64_t q = npix - p;
+        const int64_t r = (1 + isqrt64(2 * q - 1)) / 2;
+        iphi = 4 * r + 1 - (q - 2 * r * (r - 1));
+        kshift = 0;
+        nr = r;
+        iring = 4 * n - r;
+        face = (int)(8 + (iphi - 1) / nr);
+    }
+    const int jrll = (face < 4) ? 2 : (face < 8 ? 3 : 4);
+    const int jpll = (face < 4) ? (2 * face + 1) : (face < 8 ? 2 * (face - 4) : 2 * (face - 8) + 1);
+    const int64_t irt = iring - (int64_t)jrll * n + 1;
+    int64_t ipt = 2 * iphi - (int64_t)jpll * nr - kshift - 1;
+    if (ipt >= 2 * n) ipt -= 8 * n;
+    const int64_t ix = (ipt - irt) >> 1, iy = (-ipt - irt) >> 1;
+    int L = 0;
+    while ((int64_t(1) << L) < n) ++L;
+    return ((int64_t)face << (2 * L)) | (int64_t)(spread_bits((uint32_t)ix) | (spread_bits((uint32_t)iy) << 1));
 }
 
 // ---------------------------------------------------------- Catmull-Rom ----

@@ -23,6 +23,7 @@
 #include "../../include/srt.h"
 #include "srt_adjoint.hpp"
 #include "srt_device.hpp"
+#include "srt_order.hpp"
 
 namespace {
 
@@ -160,7 +161,8 @@ struct TraceArgs {
     double obs[3];
     float* out;
     double* state;
-    int xcd_groups;  // see tile_of_block
+    int xcd_groups;             // see tile_of_block
+    const uint32_t* lane_slot;  // sorted lane -> slot map (srt::order_lanes), or null
 };
 
 // Ray tile traced by workgroup `b` of `nb`.  The dispatcher hands workgroups
@@ -321,17 +323,18 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void trace_kernel(TraceAr
     const int64_t lane_ray = tile_of_block(blockIdx.x, gridDim.x, A.xcd_groups) * kBlock + threadIdx.x;
     if (lane_ray >= A.count) return;
     int64_t pix, slot;
-    if (A.order == SRT_ORDER_NESTED) {
+    const bool nested_sky = A.order == SRT_ORDER_NESTED && !A.lane_slot;
+    if (nested_sky) {  // full sky, power-of-two Nside: lane = NESTED pixel
         pix = srt::nest_to_ring(A.nside, lane_ray);
         slot = pix;
     } else {
-        slot = lane_ray;
-        pix = A.pix_list ? A.pix_list[lane_ray] : A.pix_begin + lane_ray;
+        slot = A.lane_slot ? (int64_t)A.lane_slot[lane_ray] : lane_ray;
+        pix = A.pix_list ? A.pix_list[slot] : A.pix_begin + slot;
     }
     if (pix < 0 || pix >= 12 * A.nside * A.nside) {
         // invalid entry of a caller's pixel list: NaN maps in its slot (in NESTED
-        // order the slot is the pixel itself, so nothing can be written)
-        if (A.order != SRT_ORDER_NESTED) {
+        // full-sky order the slot is the pixel itself, so nothing can be written)
+        if (!nested_sky) {
             const float nan = __builtin_nanf("");
             for (int m = 0; m < 6; ++m) A.out[m * A.count + slot] = nan;
         }
@@ -342,46 +345,64 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void trace_kernel(TraceAr
 
 // ---------------------------------------------------------------- batch ----
 // Many snapshots / observers / pixel ranges / step counts in one persistent
-// launch.  Work items ("tiles") of `tile_rays` consecutive rays of one
-// instance are handed out by an atomic counter, one wavefront at a time, so
+// launch.  Work items ("tiles") of `tile_rays` consecutive lanes of one
+// instance are handed out by atomic counters, one wavefront at a time, so
 // instances of very different cost (grid size, step count) balance across the
-// chip; every wavefront leaves when the counter passes the last tile.
+// chip.  The tiles are split into `groups` contiguous ranges with a counter
+// each; a wavefront of workgroup b starts on range b mod groups (with 8
+// groups, the range of its XCD, so neighbouring tiles share an L2), then
+// takes tiles of the other ranges once its own is drained.  Every wavefront
+// leaves after finding all ranges drained.
 struct BatchInst {
     srt::Grids G;
     int64_t nside, pix_begin, count;
     int n_steps;
     double obs[3];
     float* out;
-    int64_t tile_begin;  // first tile index of this instance
+    int64_t tile_begin;         // first tile index of this instance
+    const uint32_t* lane_slot;  // sorted lane -> slot map, or null (lane i traces slot i)
 };
 
-__global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0ull; }
+constexpr int kMaxTileGroups = 64;
+
+__global__ void reset_counters_kernel(unsigned long long* counters, int n) {
+    if ((int)threadIdx.x < n) counters[threadIdx.x] = 0ull;
+}
 
 template <class F>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchInst* __restrict__ inst, int n_inst,
-                                                       int64_t total_tiles, int tile_rays,
-                                                       unsigned long long* __restrict__ counter) {
+__global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void batch_kernel(const BatchInst* __restrict__ inst,
+                                                                         int n_inst, int64_t total_tiles,
+                                                                         int tile_rays, int groups,
+                                                                         unsigned long long* __restrict__ counters) {
     const int lane = threadIdx.x & 63;
-    for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1ull);
-        const uint32_t lo32 = __builtin_amdgcn_readfirstlane((uint32_t)t);
-        const uint32_t hi32 = __builtin_amdgcn_readfirstlane((uint32_t)(t >> 32));
-        const int64_t tile = (int64_t)(((uint64_t)hi32 << 32) | lo32);
-        if (tile >= total_tiles) break;
-        int lo = 0, hi = n_inst - 1;  // last instance with tile_begin <= tile
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (inst[mid].tile_begin <= tile)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const BatchInst& I = inst[lo];
-        const int64_t r0 = (tile - I.tile_begin) * tile_rays;
-        for (int c = 0; c < tile_rays; c += 64) {
-            const int64_t r = r0 + c + lane;
-            if (r < I.count) trace_ray<F>(I.G, I.nside, I.pix_begin + r, r, I.count, I.n_steps, I.obs, I.out, nullptr);
+    const int home = (int)(blockIdx.x % (unsigned)groups);
+    for (int r = 0; r < groups; ++r) {
+        const int x = (home + r) % groups;
+        const int64_t lo = total_tiles * x / groups, hi = total_tiles * (x + 1) / groups;
+        for (;;) {
+            unsigned long long t = 0;
+            if (lane == 0) t = atomicAdd(counters + x, 1ull);
+            const uint32_t lo32 = __builtin_amdgcn_readfirstlane((uint32_t)t);
+            const uint32_t hi32 = __builtin_amdgcn_readfirstlane((uint32_t)(t >> 32));
+            const int64_t tile = lo + (int64_t)(((uint64_t)hi32 << 32) | lo32);
+            if (tile >= hi) break;
+            int a = 0, b = n_inst - 1;  // last instance with tile_begin <= tile
+            while (a < b) {
+                const int mid = (a + b + 1) >> 1;
+                if (inst[mid].tile_begin <= tile)
+                    a = mid;
+                else
+                    b = mid - 1;
+            }
+            const BatchInst& I = inst[a];
+            const int64_t r0 = (tile - I.tile_begin) * tile_rays;
+            for (int c = 0; c < tile_rays; c += 64) {
+                const int64_t l = r0 + c + lane;
+                if (l < I.count) {
+                    const int64_t slot = I.lane_slot ? (int64_t)I.lane_slot[l] : l;
+                    trace_ray<F>(I.G, I.nside, I.pix_begin + slot, slot, I.count, I.n_steps, I.obs, I.out, nullptr);
+                }
+            }
         }
     }
 }
@@ -617,6 +638,10 @@ struct srt_context {
     // workgroup -> tile grouping of trace launches (tile_of_block); from
     // SRT_XCD_GROUPS at context creation, default 8 (the XCDs of an MI355X)
     int xcd_groups = 8;
+    // sorted lane order of partial-sky NESTED traces: lane -> slot map and the
+    // sort's work space, grown on demand
+    void* order_buf = nullptr;
+    size_t order_bytes = 0;
 };
 
 namespace {
@@ -737,6 +762,7 @@ int srt_destroy(srt_context* ctx) {
     if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
     if (ctx->grids64) (void)hipFree(ctx->grids64);
     free_records(ctx);
+    if (ctx->order_buf) (void)hipFree(ctx->order_buf);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     delete ctx;
@@ -861,18 +887,35 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: count out of range");
     if (!pix_list && (pix_begin < 0 || pix_begin + count > npix))
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: pixel range outside the sky");
-    if (ray_order == SRT_ORDER_NESTED) {
-        if (pix_list || pix_begin != 0 || count != npix)
-            return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: NESTED lane order needs the full sky");
-    } else if (ray_order != SRT_ORDER_LINEAR) {
+    if (ray_order != SRT_ORDER_NESTED && ray_order != SRT_ORDER_LINEAR)
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown ray order");
-    }
     SRT_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    // NESTED over the full sky at power-of-two Nside: lanes are NESTED pixels;
+    // any other NESTED request sorts the rays by srt::ring_order_key
+    const bool sorted = ray_order == SRT_ORDER_NESTED && count > 0 &&
+                        (pix_list || pix_begin != 0 || count != npix || (nside & (nside - 1)) != 0);
+    uint32_t* lane_slot = nullptr;
+    if (sorted) {
+        if (count > srt::kMaxOrderedRays)
+            return fail(SRT_ERR_UNSUPPORTED, "srt_trace: NESTED lane order over more than 2^31 - 1 rays");
+        const size_t map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
+        const size_t work = srt::order_work_bytes(nside, count);
+        if (!work) return fail(SRT_ERR_DEVICE, "srt_trace: lane-order work size");
+        if (map_bytes + work > ctx->order_bytes) {
+            SRT_HIP(hipDeviceSynchronize());  // an earlier launch may still read the old map
+            if (ctx->order_buf) SRT_HIP(hipFree(ctx->order_buf));
+            ctx->order_buf = nullptr;
+            ctx->order_bytes = 0;
+            SRT_HIP(hipMalloc(&ctx->order_buf, map_bytes + work));
+            ctx->order_bytes = map_bytes + work;
+        }
+        lane_slot = static_cast<uint32_t*>(ctx->order_buf);
+    }
     TraceArgs A;
     A.G.phi = ctx->grids;
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
     for (int a = 0; a < 3; ++a) A.G.b[a] = ctx->grids + (size_t)(1 + a) * pvox;
-    hipStream_t st = (hipStream_t)stream;
     if (precision == SRT_PREC_FP64) {
         const int rc = ensure_grids64(ctx, st);
         if (rc != SRT_OK) return rc;
@@ -892,7 +935,13 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     A.out = out;
     A.state = final_state;
     A.xcd_groups = ctx->xcd_groups;
+    A.lane_slot = lane_slot;
     SRT_HIP(hipEventRecord(ctx->ev_start, st));
+    if (lane_slot) {  // the sort is part of the timed trace
+        const size_t map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
+        SRT_HIP(srt::order_lanes(nside, pix_begin, count, pix_list, lane_slot,
+                                 static_cast<char*>(ctx->order_buf) + map_bytes, ctx->order_bytes - map_bytes, st));
+    }
     if (count > 0) {
         const int64_t blocks = (count + kBlock - 1) / kBlock;
         switch (precision) {
@@ -1011,7 +1060,12 @@ struct srt_plan {
     int64_t total_rays = 0, total_ray_steps = 0;
     std::vector<srt_batch_item> items;
     BatchInst* d_inst = nullptr;
-    unsigned long long* d_counter = nullptr;
+    unsigned long long* d_counter = nullptr;  // kMaxTileGroups tile counters
+    int groups = 8;                            // tile ranges (see batch_kernel)
+    int order = SRT_ORDER_NESTED;              // lane order within each item
+    void* order_buf = nullptr;                 // per-item lane -> slot maps + sort space
+    size_t order_bytes = 0;
+    std::vector<const uint32_t*> item_map;     // per item: its map in order_buf, or null
     int tile_rays = 256;
     int blocks = 0;
     int64_t total_tiles = 0;
@@ -1030,6 +1084,40 @@ void plan_drop_graph(srt_plan* p) {
     p->exec = nullptr;
     p->graph = nullptr;
     p->graph_stream = nullptr;
+}
+
+// Per-item lane -> slot maps of a NESTED-ordered plan (srt::order_lanes),
+// computed once per plan and order.
+int plan_build_orders(srt_plan* p) {
+    SRT_HIP(hipSetDevice(p->device));
+    SRT_HIP(hipDeviceSynchronize());
+    if (p->order_buf) SRT_HIP(hipFree(p->order_buf));
+    p->order_buf = nullptr;
+    p->order_bytes = 0;
+    p->item_map.assign(p->n_inst, nullptr);
+    if (p->order != SRT_ORDER_NESTED) return SRT_OK;
+    std::vector<size_t> off(p->n_inst, 0);
+    size_t maps = 0, work = 0;
+    for (int i = 0; i < p->n_inst; ++i) {
+        const srt_batch_item& it = p->items[i];
+        if (it.count <= 0 || it.count > srt::kMaxOrderedRays) continue;
+        off[i] = maps + 1;  // + 1: 0 marks an item without a map
+        maps += ((size_t)it.count * sizeof(uint32_t) + 255) / 256 * 256;
+        work = std::max(work, srt::order_work_bytes(it.nside, it.count));
+    }
+    if (!maps) return SRT_OK;
+    SRT_HIP(hipMalloc(&p->order_buf, maps + work));
+    p->order_bytes = maps + work;
+    char* base = static_cast<char*>(p->order_buf);
+    for (int i = 0; i < p->n_inst; ++i) {
+        if (!off[i]) continue;
+        const srt_batch_item& it = p->items[i];
+        uint32_t* map = reinterpret_cast<uint32_t*>(base + off[i] - 1);
+        SRT_HIP(srt::order_lanes(it.nside, it.pix_begin, it.count, nullptr, map, base + maps, work, nullptr));
+        p->item_map[i] = map;
+    }
+    SRT_HIP(hipDeviceSynchronize());
+    return SRT_OK;
 }
 
 // (re)builds the device instance table for the plan's tile size
@@ -1060,6 +1148,7 @@ int plan_build_table(srt_plan* p) {
         b.n_steps = it.n_steps;
         for (int k = 0; k < 3; ++k) b.obs[k] = it.observer_set ? it.observer[k] : 0.5 * (double)c->n;
         b.out = it.out;
+        b.lane_slot = p->item_map.empty() ? nullptr : p->item_map[i];
         b.tile_begin = tiles;
         tiles += (it.count + p->tile_rays - 1) / p->tile_rays;
     }
@@ -1071,25 +1160,25 @@ int plan_build_table(srt_plan* p) {
 }
 
 int plan_enqueue(srt_plan* p, hipStream_t st) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p->d_counter);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, p->d_counter, p->groups);
     switch (p->precision) {
         case SRT_PREC_FP64:
             hipLaunchKernelGGL(batch_kernel<srt::FormFP64>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormFP64>(),
-                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->d_counter);
+                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->groups, p->d_counter);
             break;
         case SRT_PREC_MIXED:
             if (p->bicubic)
                 hipLaunchKernelGGL(batch_kernel<srt::FormMixedBicubic>, dim3(p->blocks), dim3(kBlock),
                                    lds_bytes<srt::FormMixedBicubic>(), st, p->d_inst, p->n_inst, p->total_tiles,
-                                   p->tile_rays, p->d_counter);
+                                   p->tile_rays, p->groups, p->d_counter);
             else
                 hipLaunchKernelGGL(batch_kernel<srt::FormMixed>, dim3(p->blocks), dim3(kBlock),
                                    lds_bytes<srt::FormMixed>(), st, p->d_inst, p->n_inst, p->total_tiles,
-                                   p->tile_rays, p->d_counter);
+                                   p->tile_rays, p->groups, p->d_counter);
             break;
         default:
             hipLaunchKernelGGL(batch_kernel<srt::FormRef>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormRef>(),
-                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->d_counter);
+                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->groups, p->d_counter);
             break;
     }
     SRT_HIP(hipGetLastError());
@@ -1134,7 +1223,7 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
     }
     p->blocks = 2 * device_cus(device);
     hipError_t e = hipMalloc(&p->d_inst, sizeof(BatchInst) * n_items);
-    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, kMaxTileGroups * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipEventCreate(&p->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&p->ev_stop);
     if (e != hipSuccess) {
@@ -1142,7 +1231,8 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
         return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
                     std::string("srt_plan_create: ") + hipGetErrorName(e));
     }
-    const int rc = plan_build_table(p);
+    int rc = plan_build_orders(p);
+    if (rc == SRT_OK) rc = plan_build_table(p);
     if (rc != SRT_OK) {
         srt_plan_destroy(p);
         return rc;
@@ -1151,12 +1241,31 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
     return SRT_OK;
 }
 
+int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_set_order: null plan");
+    if (ray_order != SRT_ORDER_LINEAR && ray_order != SRT_ORDER_NESTED)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_set_order: unknown ray order");
+    if (tile_groups < 1 || tile_groups > kMaxTileGroups)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_set_order: tile_groups must be in [1, 64]");
+    SRT_HIP(hipSetDevice(plan->device));
+    SRT_HIP(hipDeviceSynchronize());
+    plan_drop_graph(plan);
+    plan->groups = tile_groups;
+    if (ray_order != plan->order || plan->item_map.size() != (size_t)plan->n_inst) {
+        plan->order = ray_order;
+        const int rc = plan_build_orders(plan);
+        if (rc != SRT_OK) return rc;
+    }
+    return plan_build_table(plan);
+}
+
 int srt_plan_destroy(srt_plan* plan) {
     if (!plan) return SRT_OK;
     (void)hipSetDevice(plan->device);
     plan_drop_graph(plan);
     if (plan->d_inst) (void)hipFree(plan->d_inst);
     if (plan->d_counter) (void)hipFree(plan->d_counter);
+    if (plan->order_buf) (void)hipFree(plan->order_buf);
     if (plan->ev_start) (void)hipEventDestroy(plan->ev_start);
     if (plan->ev_stop) (void)hipEventDestroy(plan->ev_stop);
     delete plan;

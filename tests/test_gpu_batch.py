@@ -77,6 +77,34 @@ def test_plan_direct_and_graph_replay(srt, instances, prec):
     plan.close()
 
 
+@pytest.mark.parametrize("order,groups", [("linear", 1), ("linear", 8), ("nested", 1), ("nested", 3),
+                                          ("nested", 64)])
+def test_plan_lane_order_and_tile_groups(srt, instances, order, groups):
+    import torch
+    print(f"[gpu] batch plan order {order}, {groups} tile groups", flush=True)
+    ref = expected(srt, instances, srt.PREC_MIXED)
+    plan = srt.Plan(instances, precision=srt.PREC_MIXED)
+    plan.set_order(srt.ORDER_NESTED if order == "nested" else srt.ORDER_LINEAR, groups)
+    s = torch.cuda.Stream()
+    for graph in (False, True, True):
+        for it in instances:
+            it["out"].zero_()
+        torch.cuda.synchronize()
+        plan.launch(stream=s, graph=graph)
+        s.synchronize()
+        check(instances, ref)
+    plan.configure(64, 3)  # fewer blocks than tile groups: every range is still drained
+    for it in instances:
+        it["out"].zero_()
+    torch.cuda.synchronize()
+    plan.launch(stream=s)
+    s.synchronize()
+    check(instances, ref)
+    with pytest.raises(srt.SrtError):
+        plan.set_order(srt.ORDER_NESTED, 65)
+    plan.close()
+
+
 def test_plan_autotune(srt, instances):
     import torch
     ref = expected(srt, instances, srt.PREC_MIXED)
