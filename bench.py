@@ -97,8 +97,9 @@ def synth(fields, n, seed, dev):
 
 def workload_large(args, pkg, fields, dev):
     """1024^3 potential + 3 x 1024^3 shift, Nside 2048 full sky, 512 steps, mixed, one launch per
-    layout and lane order: bicubic cell records in 8x8 column bricks and in row-major columns (when
-    they fit in device memory), then the padded grids."""
+    layout and lane order: bicubic cell records in 8x8 column bricks (also as the 12 RING batches in
+    one persistent plan launch) and in row-major columns (when they fit in device memory), then the
+    padded grids."""
     import torch
     n, nside, n_steps = 1024, 2048, 512
     rays = 12 * nside * nside
@@ -129,6 +130,20 @@ def workload_large(args, pkg, fields, dev):
     try:
         tr.set_layout(pkg.LAYOUT_BICUBIC, 8)
         run("bicubic_brick8", both)
+        # the 12 RING batches of the sky as 12 items of one persistent launch (default configuration:
+        # 256-ray tiles, 2 blocks per CU, NESTED lanes within each batch, 8 XCD tile ranges)
+        per = nside * nside
+        items = [dict(tracer=tr, nside=nside, pix_begin=b * per, count=per, n_steps=n_steps,
+                      out=out[6 * per * b: 6 * per * (b + 1)]) for b in range(12)]
+        plan = pkg.Plan(items, precision=pkg.PREC_MIXED)
+        plan.launch(stream=stream)
+        ms = plan.last_ms()
+        res["bicubic_brick8"]["plan_12_batches"] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0),
+                                                    "config": plan.config()[:2]}
+        print(f"[bench] large bicubic_brick8, 12 batches in one persistent launch: {ms:.0f} ms",
+              file=sys.stderr, flush=True)
+        plan.close()
+        del items, plan
         tr.set_layout(pkg.LAYOUT_BICUBIC, 1)
         run("bicubic_brick1", (("nested", pkg.ORDER_NESTED),))
     except pkg.SrtError as e:
@@ -417,6 +432,27 @@ def main():
         print(f"[bench] {name}: {ms_per_step:.1f} ms/step, kernel {avg_kernel_ms:.1f} ms, "
               f"{results[name]['ray_steps_per_s']:.3e} ray-steps/s", file=sys.stderr, flush=True)
 
+    # the drop-in path with host buffers (what the CLI does after reading HDF5): host -> device upload
+    # with the record build, then a synchronous trace with host outputs (device -> host copy of the maps)
+    end_to_end = None
+    if "mixed" in results:
+        phi_h = phi.cpu().numpy()
+        shift_h = fields.interleave_shift(shift).cpu().numpy()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr.upload(phi_h, shift_h)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        maps_h = tr.trace(nside, n_steps, precision=pkg.PREC_MIXED, order=order)
+        t2 = time.perf_counter()
+        end_to_end = {"upload_from_host_s": t1 - t0, "trace_to_host_s": t2 - t1,
+                      "kernel_ms": tr.last_trace_ms(),
+                      "ray_steps_per_s_incl_transfers": rays * n_steps / (t2 - t0),
+                      "maps_equal_device_run": bool(np.array_equal(maps_h[:, subset], maps_sub["mixed"]))}
+        print(f"[bench] end to end from host buffers: upload {t1 - t0:.2f} s, trace {t2 - t1:.2f} s",
+              file=sys.stderr, flush=True)
+        del phi_h, shift_h, maps_h
+
     cpu_baseline = None
     accuracy = {}
     if not args.no_oracle:
@@ -482,6 +518,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "upload_s": upload_s,
+        "end_to_end_host_buffers": end_to_end,
     }
     print(json.dumps(line))
     tr.close()

@@ -209,8 +209,10 @@ int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups);
 int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks);
 int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps);
 
-/* Times every (tile_rays, blocks per CU) candidate on `stream` and keeps the
- * fastest; returns the choice. */
+/* Times every (tile ranges, tile_rays, blocks per CU) candidate (1 or 8
+ * ranges, 64..1024 rays, 1/2/4 blocks per CU) on `stream` and keeps the
+ * fastest; returns the tile size and block count (srt_plan_get_config and
+ * the plan keep the range count). */
 int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks);
 
 /* Launches the plan on `stream`.  use_graph = 1 captures the launch sequence

@@ -1340,25 +1340,32 @@ int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks)
     const int cus = device_cus(plan->device);
     const int tiles[] = {64, 128, 256, 512, 1024};
     const int per_cu[] = {1, 2, 4};
+    const int group_counts[] = {1, 8};
     float best = 0.0f;
-    int bt = plan->tile_rays, bb = plan->blocks;
+    int bt = plan->tile_rays, bb = plan->blocks, bg = plan->groups;
     bool have = false;
-    for (int t : tiles)
-        for (int w : per_cu) {
-            int rc = srt_plan_configure(plan, t, w * cus);
-            if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);  // warm-up
-            float ms = 0.0f;
-            if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);
-            if (rc == SRT_OK) rc = srt_plan_last_ms(plan, &ms);
-            if (rc != SRT_OK) return rc;
-            if (!have || ms < best) {
-                best = ms;
-                bt = t;
-                bb = w * cus;
-                have = true;
+    for (int g : group_counts) {
+        int rc = srt_plan_set_order(plan, plan->order, g);
+        if (rc != SRT_OK) return rc;
+        for (int t : tiles)
+            for (int w : per_cu) {
+                rc = srt_plan_configure(plan, t, w * cus);
+                if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);  // warm-up
+                float ms = 0.0f;
+                if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);
+                if (rc == SRT_OK) rc = srt_plan_last_ms(plan, &ms);
+                if (rc != SRT_OK) return rc;
+                if (!have || ms < best) {
+                    best = ms;
+                    bt = t;
+                    bb = w * cus;
+                    bg = g;
+                    have = true;
+                }
             }
-        }
-    const int rc = srt_plan_configure(plan, bt, bb);
+    }
+    int rc = srt_plan_set_order(plan, plan->order, bg);
+    if (rc == SRT_OK) rc = srt_plan_configure(plan, bt, bb);
     if (rc != SRT_OK) return rc;
     if (tile_rays) *tile_rays = bt;
     if (blocks) *blocks = bb;
