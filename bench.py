@@ -131,7 +131,7 @@ def workload_large(args, pkg, fields, dev):
         tr.set_layout(pkg.LAYOUT_BICUBIC, 8)
         run("bicubic_brick8", both)
         # the 12 RING batches of the sky as 12 items of one persistent launch (default configuration:
-        # 256-ray tiles, 2 blocks per CU, NESTED lanes within each batch, 8 XCD tile ranges)
+        # 64-ray tiles, 2 blocks per CU, NESTED lanes within each batch, 8 XCD tile ranges)
         per = nside * nside
         items = [dict(tracer=tr, nside=nside, pix_begin=b * per, count=per, n_steps=n_steps,
                       out=out[6 * per * b: 6 * per * (b + 1)]) for b in range(12)]

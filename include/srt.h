@@ -89,7 +89,10 @@ typedef enum {
 } srt_ray_order;
 
 /* Replaces the device selection and allocation bookkeeping of main()
- * (.cu:690-772).  device = HIP device ordinal. */
+ * (.cu:690-772).  device = HIP device ordinal.  Environment read here:
+ * SRT_MAX_ALLOC_BYTES (largest single device allocation the context may make;
+ * larger requests fail with SRT_ERR_OUT_OF_MEMORY, for exercising those paths)
+ * and SRT_XCD_GROUPS (workgroup grouping of trace launches, default 8). */
 int srt_create(int device, srt_context** ctx);
 
 /* Frees every device buffer of the context (.cu:917-920, 989-998). */
@@ -110,7 +113,10 @@ const char* srt_last_error(void);
  *   shift_format: srt_shift_format
  * The grid must be cubic (nx == ny == nz, 4 <= nx <= 2048); the reference
  * silently assumes equal dimensions for both files (.cu:645, 676, 752-772).
- * Replaces any previous snapshot of the context. */
+ * Replaces any previous snapshot of the context.  If the context's layout is
+ * SRT_LAYOUT_BICUBIC and the records do not fit, the snapshot is still
+ * uploaded, the context falls back to SRT_LAYOUT_TAPS and the call returns
+ * SRT_ERR_OUT_OF_MEMORY. */
 int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64_t nx, int64_t ny, int64_t nz,
                int inputs_on_device, int shift_format);
 
@@ -205,7 +211,9 @@ int srt_plan_destroy(srt_plan* plan);
  * an L2, then helps the other ranges).  Outputs do not depend on either. */
 int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups);
 
-/* Tile size in rays (multiple of 64) and number of persistent 256-lane blocks. */
+/* Tile size in rays (multiple of 64; default 64: a wavefront claims one
+ * 64-ray group at a time) and number of persistent 256-lane blocks (default
+ * 2 per CU). */
 int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks);
 int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps);
 

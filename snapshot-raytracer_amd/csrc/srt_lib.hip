@@ -642,15 +642,38 @@ struct srt_context {
     // sort's work space, grown on demand
     void* order_buf = nullptr;
     size_t order_bytes = 0;
+    // largest single device allocation the context may make (0 = no limit);
+    // from SRT_MAX_ALLOC_BYTES at creation, so that callers and tests can
+    // exercise the out-of-memory paths on any device
+    size_t max_alloc = 0;
 };
 
 namespace {
+
+// hipMalloc within the context's allocation limit
+hipError_t ctx_malloc(const srt_context* ctx, void** p, size_t bytes) {
+    if (ctx->max_alloc && bytes > ctx->max_alloc) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+    }
+    return e;
+}
+
+template <class T>
+hipError_t ctx_malloc(const srt_context* ctx, T** p, size_t bytes) {
+    return ctx_malloc(ctx, reinterpret_cast<void**>(p), bytes);
+}
 
 // FP64 copies of the padded grids (the FP64 kernel and the adjoint read them)
 int ensure_grids64(srt_context* ctx, hipStream_t st) {
     if (ctx->grids64) return SRT_OK;
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
-    SRT_HIP(hipMalloc(&ctx->grids64, 4 * pvox * sizeof(double)));
+    SRT_HIP(ctx_malloc(ctx, &ctx->grids64, 4 * pvox * sizeof(double)));
     const int blocks = (int)std::min<int64_t>(((int64_t)(4 * pvox) + 255) / 256, 65536);
     hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, st, ctx->grids, ctx->grids64, (int64_t)(4 * pvox));
     SRT_HIP(hipGetLastError());
@@ -679,10 +702,8 @@ int build_records(srt_context* ctx) {
     const int n = ctx->n, P = ctx->pad, s = ctx->brick_shift, nb = brick_columns(n, s);
     const size_t cols = (size_t)nb * nb << (2 * s);
     const size_t field = cols * (size_t)P * 16;
-    const hipError_t ea = hipMalloc(&ctx->rec, 4 * field * sizeof(float));
+    const hipError_t ea = ctx_malloc(ctx, &ctx->rec, 4 * field * sizeof(float));
     if (ea != hipSuccess) {
-        ctx->rec = nullptr;
-        (void)hipGetLastError();  // a failed allocation must not surface at the next launch check
         return fail(ea == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
                     "bicubic records: hipMalloc of " + std::to_string(4 * field * sizeof(float)) + " bytes: " +
                         hipGetErrorName(ea));
@@ -745,6 +766,10 @@ int srt_create(int device, srt_context** ctx) {
         const int v = std::atoi(g);
         if (v >= 1 && v <= 64) c->xcd_groups = v;
     }
+    if (const char* m = std::getenv("SRT_MAX_ALLOC_BYTES")) {
+        const long long v = std::atoll(m);
+        if (v > 0) c->max_alloc = (size_t)v;
+    }
     hipError_t e1 = hipEventCreate(&c->ev_start);
     hipError_t e2 = hipEventCreate(&c->ev_stop);
     if (e1 != hipSuccess || e2 != hipSuccess) {
@@ -802,12 +827,12 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
     }
     free_records(ctx);
     float* grids = nullptr;
-    SRT_HIP(hipMalloc(&grids, 4 * pvox * sizeof(float)));
+    SRT_HIP(ctx_malloc(ctx, &grids, 4 * pvox * sizeof(float)));
     const float* dphi = phi;
     const float* dshift = shift_xyz;
     float* staging = nullptr;
     if (!inputs_on_device) {
-        hipError_t e = hipMalloc(&staging, 4 * nvox * sizeof(float));
+        hipError_t e = ctx_malloc(ctx, &staging, 4 * nvox * sizeof(float));
         if (e != hipSuccess) {
             (void)hipFree(grids);
             return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
@@ -839,7 +864,13 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
     ctx->grids = grids;
     ctx->n = n;
     ctx->pad = P;
-    if (ctx->layout == SRT_LAYOUT_BICUBIC) return build_records(ctx);
+    if (ctx->layout == SRT_LAYOUT_BICUBIC) {
+        // the snapshot is uploaded either way; without records the context falls
+        // back to the padded grids and says so
+        const int rc = build_records(ctx);
+        if (rc != SRT_OK) ctx->layout = SRT_LAYOUT_TAPS;
+        return rc;
+    }
     return SRT_OK;
 }
 
@@ -907,7 +938,7 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
             if (ctx->order_buf) SRT_HIP(hipFree(ctx->order_buf));
             ctx->order_buf = nullptr;
             ctx->order_bytes = 0;
-            SRT_HIP(hipMalloc(&ctx->order_buf, map_bytes + work));
+            SRT_HIP(ctx_malloc(ctx, &ctx->order_buf, map_bytes + work));
             ctx->order_bytes = map_bytes + work;
         }
         lane_slot = static_cast<uint32_t*>(ctx->order_buf);
@@ -998,7 +1029,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
         if (ctx->adj_buf) SRT_HIP(hipFree(ctx->adj_buf));
         ctx->adj_buf = nullptr;
         ctx->adj_bytes = 0;
-        SRT_HIP(hipMalloc(&ctx->adj_buf, need));
+        SRT_HIP(ctx_malloc(ctx, &ctx->adj_buf, need));
         ctx->adj_bytes = need;
     }
     hipStream_t st = (hipStream_t)stream;
@@ -1066,7 +1097,7 @@ struct srt_plan {
     void* order_buf = nullptr;                 // per-item lane -> slot maps + sort space
     size_t order_bytes = 0;
     std::vector<const uint32_t*> item_map;     // per item: its map in order_buf, or null
-    int tile_rays = 256;
+    int tile_rays = 64;  // one 64-ray group per claim: in-flight rays of an XCD stay as compact as a trace launch's
     int blocks = 0;
     int64_t total_tiles = 0;
     hipGraph_t graph = nullptr;
@@ -1397,9 +1428,9 @@ int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t c
         if (d_pix) (void)hipFree(d_pix);
         if (d_state) (void)hipFree(d_state);
     };
-    hipError_t e = hipMalloc(&d_out, std::max<size_t>(1, 6 * c) * sizeof(float));
-    if (e == hipSuccess && pix_list) e = hipMalloc(&d_pix, std::max<size_t>(1, c) * sizeof(int64_t));
-    if (e == hipSuccess && final_state) e = hipMalloc(&d_state, std::max<size_t>(1, 19 * c) * sizeof(double));
+    hipError_t e = ctx_malloc(ctx, &d_out, std::max<size_t>(1, 6 * c) * sizeof(float));
+    if (e == hipSuccess && pix_list) e = ctx_malloc(ctx, &d_pix, std::max<size_t>(1, c) * sizeof(int64_t));
+    if (e == hipSuccess && final_state) e = ctx_malloc(ctx, &d_state, std::max<size_t>(1, 19 * c) * sizeof(double));
     if (e == hipSuccess && pix_list && c) e = hipMemcpy(d_pix, pix_list, c * sizeof(int64_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         cleanup();

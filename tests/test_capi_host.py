@@ -46,6 +46,10 @@ def test_null_arguments_rejected_without_device_calls(srt):
     ms = ctypes.c_float()
     assert L.srt_last_trace_ms(None, ctypes.byref(ms)) == 1
     assert L.srt_grid_info(None, None, None, None, None) == 1
+    assert L.srt_set_layout(None, 1, 8) == 1
+    assert L.srt_get_layout(None, None, None, None) == 1
+    assert L.srt_plan_set_order(None, 1, 8) == 1
+    assert L.srt_plan_autotune(None, None, None, None) == 1
     assert b"null" in L.srt_last_error()
 
 

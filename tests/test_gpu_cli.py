@@ -77,3 +77,23 @@ def test_cli_reference_storage_mode(files):
     assert r.returncode == 255
     r = run(d, ["4", "6"], {"SRT_LAYOUT": "bogus"})
     assert r.returncode == 255
+
+
+def test_cli_records_that_do_not_fit(srt, files):
+    """SRT_LAYOUT=auto falls back to the padded grids when the bicubic records do
+    not fit (forced with a per-allocation limit) and says so; SRT_LAYOUT=bicubic
+    fails with exit status 255."""
+    from snapshot_raytracer_amd import snapshot_io
+    d, phi, shift, shift_il = files
+    limit = {"SRT_MAX_ALLOC_BYTES": str(4 << 20)}  # 32^3: padded grids 0.7 MB fit, records 9.2 MB do not
+    r = run(d, ["4", "6"], dict(limit, SRT_LAYOUT="auto"))
+    assert r.returncode == 0, r.stderr
+    assert "do not fit" in r.stdout
+    tr = srt.Tracer(0)
+    tr.upload(phi, shift_il)
+    ref = tr.trace(4, 6, precision=srt.PREC_MIXED)
+    tr.close()
+    got, _ = snapshot_io.read_healpix_fits(str(d / "dAmap_batch0.fits"))
+    assert np.array_equal(got, ref[0])
+    r = run(d, ["4", "6"], dict(limit, SRT_LAYOUT="bicubic"))
+    assert r.returncode == 255

@@ -149,3 +149,30 @@ def test_batch_plan_over_bicubic_contexts(srt, fields):
     plan.close()
     for it in items:
         it["tracer"].close()
+
+
+def test_record_allocation_failure_falls_back(srt, snap, monkeypatch):
+    """Records that do not fit (forced by a per-allocation limit of the context):
+    srt_set_layout and srt_upload report SRT_ERR_OUT_OF_MEMORY, the context keeps
+    the snapshot on the padded grids, and traces work and equal a padded-grid
+    trace."""
+    print("[gpu] record allocation failure", flush=True)
+    ref_tr = srt.Tracer(0)
+    ref_tr.upload(snap[0], snap[2])
+    ref = ref_tr.trace(NSIDE, STEPS, precision=srt.PREC_MIXED)
+    ref_tr.close()
+    monkeypatch.setenv("SRT_MAX_ALLOC_BYTES", str(8 << 20))  # padded grids 4.9 MB fit, records 70 MB do not
+    tr = srt.Tracer(0)
+    monkeypatch.delenv("SRT_MAX_ALLOC_BYTES")
+    tr.set_layout(srt.LAYOUT_BICUBIC, 8)  # no snapshot yet: nothing to build
+    with pytest.raises(srt.SrtError) as ei:
+        tr.upload(snap[0], snap[2])
+    assert ei.value.status == 3
+    assert tr.layout()[0] == srt.LAYOUT_TAPS and tr.layout()[2] == 0
+    assert np.array_equal(tr.trace(NSIDE, STEPS, precision=srt.PREC_MIXED), ref)
+    with pytest.raises(srt.SrtError) as ei:
+        tr.set_layout(srt.LAYOUT_BICUBIC, 8)
+    assert ei.value.status == 3
+    assert tr.layout()[0] == srt.LAYOUT_TAPS
+    assert np.array_equal(tr.trace(NSIDE, STEPS, precision=srt.PREC_MIXED), ref)
+    tr.close()
