@@ -54,9 +54,10 @@ def parse():
     p.add_argument("--nside", type=int, default=1024)
     p.add_argument("--n-steps", type=int, default=256)
     p.add_argument("--seed", type=int, default=2)
-    p.add_argument("--kernels", default="mixed,fp64,mixed_taps",
+    p.add_argument("--kernels", default="mixed,fp64,mixed_taps,ref",
                    help="comma list of: mixed (FP32 interpolation from bicubic cell records), "
-                        "mixed_taps (same arithmetic on the padded cubes), fp64, ref")
+                        "mixed_taps (same arithmetic on the padded cubes), fp64, ref (the reference GPU "
+                        "code's float/double mix, FP32 shift)")
     p.add_argument("--brick", type=int, default=8, help="column brick edge of the bicubic records")
     p.add_argument("--order", default="nested", choices=["nested", "linear"])
     p.add_argument("--subset-frac", type=float, default=0.01)
@@ -513,7 +514,7 @@ def main():
                             if "mixed" in results and "fp64" in results else None),
         "layout": {"mixed": f"bicubic cell records, {args.brick}x{args.brick} column bricks",
                    "mixed_taps": "padded periodic cubes", "fp64": "padded periodic cubes (FP64 copies)",
-                   "build_s": layout_s},
+                   "ref": "padded periodic cubes", "build_s": layout_s},
         "map_error_vs_fp64_oracle": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

@@ -198,7 +198,7 @@ typedef struct {
 typedef struct srt_plan srt_plan;
 
 /* Builds a plan for tracing all items in one persistent launch: tiles of
- * consecutive rays of one item are handed to wavefronts by an atomic counter,
+ * consecutive rays of one item are handed to workgroups by atomic counters,
  * so items of different grid size, step count and ray count share the GPU.
  * All contexts must be on one device and stay alive while the plan is used. */
 int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt_plan** plan);
@@ -211,9 +211,9 @@ int srt_plan_destroy(srt_plan* plan);
  * an L2, then helps the other ranges).  Outputs do not depend on either. */
 int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups);
 
-/* Tile size in rays (multiple of 64; default 64: a wavefront claims one
- * 64-ray group at a time) and number of persistent 256-lane blocks (default
- * 2 per CU). */
+/* Rays per wavefront per claim (multiple of 64; default 64: a 256-lane
+ * workgroup claims 4 x 64 neighbouring rays at a time) and number of
+ * persistent 256-lane workgroups (default 2 per CU). */
 int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks);
 int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps);
 

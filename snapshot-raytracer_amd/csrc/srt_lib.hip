@@ -345,14 +345,17 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void trace_kernel(TraceAr
 
 // ---------------------------------------------------------------- batch ----
 // Many snapshots / observers / pixel ranges / step counts in one persistent
-// launch.  Work items ("tiles") of `tile_rays` consecutive lanes of one
-// instance are handed out by atomic counters, one wavefront at a time, so
+// launch.  Work items ("tiles") of 4 x `tile_rays` consecutive lanes of one
+// instance are handed out by atomic counters, one workgroup at a time, so
 // instances of very different cost (grid size, step count) balance across the
-// chip.  The tiles are split into `groups` contiguous ranges with a counter
-// each; a wavefront of workgroup b starts on range b mod groups (with 8
-// groups, the range of its XCD, so neighbouring tiles share an L2), then
-// takes tiles of the other ranges once its own is drained.  Every wavefront
-// leaves after finding all ranges drained.
+// chip; wavefront w of the workgroup traces the 64-lane groups w, w + 4, ...
+// of the tile, so the four wavefronts of a CU trace neighbouring rays (with
+// NESTED lane maps, one 16 x 16 sky block per round) and share its L1, as the
+// workgroups of a single trace launch do.  The tiles are split into `groups`
+// contiguous ranges with a counter each; workgroup b starts on range
+// b mod groups (with 8 groups, the range of its XCD, so neighbouring tiles
+// share an L2), then takes tiles of the other ranges once its own is drained.
+// Every workgroup leaves after finding all ranges drained.
 struct BatchInst {
     srt::Grids G;
     int64_t nside, pix_begin, count;
@@ -374,18 +377,20 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void batch_kernel(const B
                                                                          int n_inst, int64_t total_tiles,
                                                                          int tile_rays, int groups,
                                                                          unsigned long long* __restrict__ counters) {
+    __shared__ unsigned long long claimed;
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    constexpr int kWaves = kBlock / 64;
     const int home = (int)(blockIdx.x % (unsigned)groups);
     for (int r = 0; r < groups; ++r) {
         const int x = (home + r) % groups;
         const int64_t lo = total_tiles * x / groups, hi = total_tiles * (x + 1) / groups;
         for (;;) {
-            unsigned long long t = 0;
-            if (lane == 0) t = atomicAdd(counters + x, 1ull);
-            const uint32_t lo32 = __builtin_amdgcn_readfirstlane((uint32_t)t);
-            const uint32_t hi32 = __builtin_amdgcn_readfirstlane((uint32_t)(t >> 32));
-            const int64_t tile = lo + (int64_t)(((uint64_t)hi32 << 32) | lo32);
-            if (tile >= hi) break;
+            if (threadIdx.x == 0) claimed = atomicAdd(counters + x, 1ull);
+            __syncthreads();
+            const int64_t tile = lo + (int64_t)claimed;
+            __syncthreads();  // every wavefront has read `claimed` before the next claim
+            if (tile >= hi) break;  // uniform across the workgroup
             int a = 0, b = n_inst - 1;  // last instance with tile_begin <= tile
             while (a < b) {
                 const int mid = (a + b + 1) >> 1;
@@ -395,9 +400,9 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void batch_kernel(const B
                     b = mid - 1;
             }
             const BatchInst& I = inst[a];
-            const int64_t r0 = (tile - I.tile_begin) * tile_rays;
+            const int64_t r0 = (tile - I.tile_begin) * (int64_t)tile_rays * kWaves;
             for (int c = 0; c < tile_rays; c += 64) {
-                const int64_t l = r0 + c + lane;
+                const int64_t l = r0 + (int64_t)c * kWaves + wave * 64 + lane;
                 if (l < I.count) {
                     const int64_t slot = I.lane_slot ? (int64_t)I.lane_slot[l] : l;
                     trace_ray<F>(I.G, I.nside, I.pix_begin + slot, slot, I.count, I.n_steps, I.obs, I.out, nullptr);
@@ -1181,7 +1186,8 @@ int plan_build_table(srt_plan* p) {
         b.out = it.out;
         b.lane_slot = p->item_map.empty() ? nullptr : p->item_map[i];
         b.tile_begin = tiles;
-        tiles += (it.count + p->tile_rays - 1) / p->tile_rays;
+        const int64_t per_tile = (int64_t)p->tile_rays * (kBlock / 64);
+        tiles += (it.count + per_tile - 1) / per_tile;
     }
     p->total_tiles = tiles;
     SRT_HIP(hipSetDevice(p->device));
