@@ -16,6 +16,11 @@
 namespace srt {
 namespace adj {
 
+// Global-memory pointers: loads and atomics through them are global (not
+// flat) instructions even where the compiler cannot see the pointer's origin.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
 // Catmull-Rom weights of orders 0..3 along one axis (third-derivative weights
 // are constant: d3/dt3 of the value weights).
 struct Axis4 {
@@ -52,14 +57,17 @@ __device__ __forceinline__ Interp3 tricubic3(const float* __restrict__ base, siz
                                              const Axis4& ay, const Axis4& az) {
     double o[20];
     for (int k = 0; k < 20; ++k) o[k] = 0.0;
+    // the z slices are a loop (not unrolled): the code of the reverse sweep
+    // stays a few thousand instructions, with fewer values live at once
+#pragma unroll 1
     for (int c = 0; c < 4; ++c) {
         // y-pass results for this z slice, indexed [ox][oy]
         double Y[4][4];
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) Y[i][j] = 0.0;
         for (int b = 0; b < 4; ++b) {
-            const Row4 q = load_row(base + b * py + c * pz);
-            const double p[4] = {q.x, q.y, q.z, q.w};
+            const gptr<const float> row = (gptr<const float>)(base + b * py + c * pz);
+            const double p[4] = {row[0], row[1], row[2], row[3]};
             double X[4];
             for (int ox = 0; ox < 4; ++ox)
                 X[ox] = ax.w[ox][0] * p[0] + ax.w[ox][1] * p[1] + ax.w[ox][2] * p[2] + ax.w[ox][3] * p[3];
@@ -106,6 +114,7 @@ __device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, siz
                          __builtin_amdgcn_readfirstlane((int)(off >> 32)) == (int)(off >> 32);
     const bool all_uniform = __all(uniform);
     // z-pass transpose: per slice c, adjoints of the six y-pass sums
+#pragma unroll 1
     for (int c = 0; c < 4; ++c) {
         const double yv = az.w[0][c] * bar[0] + az.w[1][c] * bar[3] + az.w[2][c] * bar[9];
         const double yd = az.w[0][c] * bar[2] + az.w[1][c] * bar[8];
@@ -119,12 +128,12 @@ __device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, siz
             const double rdd = ay.w[0][b] * xdd;
             for (int a = 0; a < 4; ++a) {
                 double pb = ax.w[0][a] * rv + ax.w[1][a] * rd + ax.w[2][a] * rdd;
-                double* dst = gbase_dev + off + a + b * py + c * pz;
+                const gptr<double> dst = (gptr<double>)(gbase_dev + off + a + b * py + c * pz);
                 if (all_uniform) {
                     for (int m = 32; m >= 1; m >>= 1) pb += __shfl_xor(pb, m);
-                    if ((threadIdx.x & 63) == 0) atomicAdd(dst, pb);
+                    if ((threadIdx.x & 63) == 0) (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, pb);
                 } else {
-                    atomicAdd(dst, pb);
+                    (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, pb);
                 }
             }
         }
@@ -145,9 +154,12 @@ struct RhsTape {
     bool br;
 };
 
+// One field's stencil: offset of tap (-1,-1,-1) in the padded grid and the
+// cell offsets along x, y, z.  The Catmull-Rom weights are recomputed where
+// they are used, so that four stencils do not hold 4 x 48 weights live.
 struct Stencil {
-    size_t off;       // offset of tap (-1,-1,-1) in the padded grid
-    Axis4 ax, ay, az;
+    size_t off;
+    double t[3];
 };
 
 __device__ __forceinline__ void locate_stencils(const Grids& G, const double x[3], Stencil st[4]) {
@@ -170,16 +182,18 @@ __device__ __forceinline__ void locate_stencils(const Grids& G, const double x[3
         const int iy = (a == 1) ? i1[1] : i0[1];
         const int iz = (a == 2) ? i1[2] : i0[2];
         st[f].off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
-        st[f].ax = cr_axis4(a == 0 ? t1[0] : t0[0]);
-        st[f].ay = cr_axis4(a == 1 ? t1[1] : t0[1]);
-        st[f].az = cr_axis4(a == 2 ? t1[2] : t0[2]);
+        st[f].t[0] = a == 0 ? t1[0] : t0[0];
+        st[f].t[1] = a == 1 ? t1[1] : t0[1];
+        st[f].t[2] = a == 2 ? t1[2] : t0[2];
     }
 }
 
-// Right-hand side at stage state s (19, FP64), keeping the intermediates and
-// the third derivatives of the interpolations (t3[field][10]).
+// Right-hand side at stage state s (19, FP64), keeping the intermediates.
+// The interpolations read the FP64 copies of the grids (value, gradient,
+// Hessian); the third derivatives the position adjoint needs are
+// re-interpolated in rhs_vjp rather than kept live from here.
 __device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, const Stencil st[4], RhsTape& T,
-                                            double t3[4][10], double dy[19]) {
+                                            double dy[19]) {
     for (int i = 0; i < 3; ++i) {
         T.nd[i] = s[3 + i];
         T.e1d[i] = s[6 + i];
@@ -200,13 +214,15 @@ __device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, con
         T.e2[i] = T.e2d[i] * T.r2;
     }
     const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
-    const float* grids[4] = {G.phi, G.b[0], G.b[1], G.b[2]};
+    const double* grids[4] = {G.phi64, G.b64[0], G.b64[1], G.b64[2]};
     for (int f = 0; f < 4; ++f) {
-        const Interp3 r = tricubic3(grids[f] + st[f].off, py, pz, st[f].ax, st[f].ay, st[f].az);
-        for (int k = 0; k < 10; ++k) t3[f][k] = r.t[k];
-        double H[3][3];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) H[i][j] = r.h[h6(i, j)];
+        const Axis<double> wx = cr_axis<double>(st[f].t[0]), wy = cr_axis<double>(st[f].t[1]),
+                           wz = cr_axis<double>(st[f].t[2]);
+        const Interp<double> q = tricubic<double>(grids[f] + st[f].off, py, pz, wx, wy, wz);
+        const struct {
+            double v, g[3];
+        } r = {q.v, {q.gx, q.gy, q.gz}};
+        const double H[3][3] = {{q.hxx, q.hxy, q.hxz}, {q.hxy, q.hyy, q.hyz}, {q.hxz, q.hyz, q.hzz}};
         if (f == 0) {
             T.phi = r.v;
             for (int i = 0; i < 3; ++i) {
@@ -305,9 +321,8 @@ __device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, con
 
 // Vector-Jacobian product of the right-hand side: given a = dL/d(dy) (19),
 // returns sbar = dL/ds (19) and scatters dL/d(grid) for the four fields.
-__device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], const RhsTape& T,
-                                        const double t3[4][10], const double a[19], double* const gdst[4],
-                                        double sbar[19]) {
+__device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], const RhsTape& T, const double a[19],
+                                        double* const gdst[4], double sbar[19]) {
     const double* n = T.n;
     const double* e1 = T.e1;
     const double* e2 = T.e2;
@@ -506,15 +521,18 @@ __device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], con
             bar[9] = Hbb[aa][2][2];
         }
         // position: d(value)/dx = g, d(g_i)/dx_k = H_ik, d(h_u)/dx_k = third derivative
+        const Axis4 ax = cr_axis4(st[f].t[0]), ay = cr_axis4(st[f].t[1]), az = cr_axis4(st[f].t[2]);
+        const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
+        const float* grids[4] = {G.phi, G.b[0], G.b[1], G.b[2]};
+        const Interp3 r3 = tricubic3(grids[f] + st[f].off, py, pz, ax, ay, az);
         static const int hu[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
         for (int k = 0; k < 3; ++k) {
             double acc = bar[0] * gcur[k];
             for (int i = 0; i < 3; ++i) acc += bar[1 + i] * Hcur[i][k];
-            for (int u = 0; u < 6; ++u) acc += bar[4 + u] * t3[f][t3idx(hu[u][0], hu[u][1], k)];
+            for (int u = 0; u < 6; ++u) acc += bar[4 + u] * r3.t[t3idx(hu[u][0], hu[u][1], k)];
             xb[k] += acc;
         }
-        scatter_taps(gdst[f], st[f].off, (size_t)G.pad, (size_t)G.pad * (size_t)G.pad, st[f].ax, st[f].ay,
-                     st[f].az, bar);
+        scatter_taps(gdst[f], st[f].off, py, pz, ax, ay, az, bar);
     }
 
     // normalisations: n = nd / |nd| etc.

@@ -274,14 +274,16 @@ __device__ __forceinline__ int locate(double x, int n, double inv_n, double& t) 
     return r;
 }
 
+// Grid rows are always in global memory; the explicit address space keeps the
+// loads global (not flat) also in functions that are not inlined into a
+// kernel, where the compiler cannot see where a pointer comes from.
 struct alignas(4) Row4 {
     float x, y, z, w;
 };
 
 __device__ __forceinline__ Row4 load_row(const float* p) {
-    Row4 r;
-    __builtin_memcpy(&r, p, sizeof(Row4));
-    return r;
+    const __attribute__((address_space(1))) float* g = (const __attribute__((address_space(1))) float*)p;
+    return Row4{g[0], g[1], g[2], g[3]};
 }
 
 struct alignas(8) Row4d {
@@ -289,9 +291,8 @@ struct alignas(8) Row4d {
 };
 
 __device__ __forceinline__ Row4d load_row(const double* p) {
-    Row4d r;
-    __builtin_memcpy(&r, p, sizeof(Row4d));
-    return r;
+    const __attribute__((address_space(1))) double* g = (const __attribute__((address_space(1))) double*)p;
+    return Row4d{g[0], g[1], g[2], g[3]};
 }
 
 // value, gradient (x, y, z), Hessian (xx, xy, xz, yy, yz, zz)

@@ -463,16 +463,35 @@ __device__ __forceinline__ void rk4_step(const srt::Grids& G, double* y) {
     for (int j = 0; j < 19; ++j) y[j] += (acc[j] + k[j]) * sixth;
 }
 
-// dL/ds of one stage given dL/dk, scattering the grid gradient
-__device__ __forceinline__ void stage_vjp(const AdjArgs& A, const double* s, const double* kbar, double* sbar) {
-    srt::adj::Stencil st[4];
-    srt::adj::locate_stencils(A.G, s, st);
-    srt::adj::RhsTape T;
-    double t3[4][10];
-    double dy[19];
-    srt::adj::rhs_forward(A.G, s, st, T, t3, dy);
+// Per-lane vectors of the reverse sweep, in LDS (one column per lane, no
+// sharing, no barriers): dL/dy of the step being reversed, its running sum
+// over the stages, and the stage VJP's input dL/dk and output dL/ds.
+enum { kLam = 0, kYbar = 1, kKbar = 2, kSbar = 3 };
+__shared__ double adj_vec[4][19][kAdjBlock];
+
+// dL/ds of one stage given dL/dk (adj_vec[kKbar]), into adj_vec[kSbar],
+// scattering the grid gradient.  The stage state is component j at
+// s[j * stride].  Not inlined: a call boundary keeps the register allocation
+// of the reverse sweep local.
+__device__ __noinline__ void stage_vjp(const AdjArgs& A, const double* __restrict__ s_in, int64_t stride) {
+    const int l = threadIdx.x;
+    double s[19], kbar[19], sbar[19];
+    const srt::adj::gptr<const double> sg = (srt::adj::gptr<const double>)s_in;
+    for (int j = 0; j < 19; ++j) {
+        s[j] = sg[j * stride];
+        kbar[j] = adj_vec[kKbar][j][l];
+    }
+    // the grids and gradient pointers, read once (A is not re-read after the
+    // gradient atomics, which the compiler cannot prove do not alias it)
+    const srt::Grids G = A.G;
     double* const gd[4] = {A.grad[0], A.grad[1], A.grad[2], A.grad[3]};
-    srt::adj::rhs_vjp(A.G, st, T, t3, kbar, gd, sbar);
+    srt::adj::Stencil st[4];
+    srt::adj::locate_stencils(G, s, st);
+    srt::adj::RhsTape T;
+    double dy[19];
+    srt::adj::rhs_forward(G, s, st, T, dy);
+    srt::adj::rhs_vjp(G, st, T, kbar, gd, sbar);
+    for (int j = 0; j < 19; ++j) adj_vec[kSbar][j][l] = sbar[j];
 }
 
 __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
@@ -493,13 +512,17 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
                     0.0,      -1.0,     0.0,      0.0,     0.0,     0.0, 0.0};
     const int K = A.K;
     const int nseg = (A.n_steps + K - 1) / K;
-    double* ck = A.ckpt + ray * (int64_t)(nseg + 1) * 19;
+    const int64_t C = A.count;
+    // checkpoints and segment states are component-major with the ray fastest
+    // (state q, component j of this ray at [(q * 19 + j) * C + ray]), so the
+    // lanes of a wavefront read and write them coalesced
+    double* const ck = A.ckpt + ray;
     for (int sgi = 0; sgi < nseg; ++sgi) {
-        for (int j = 0; j < 19; ++j) ck[sgi * 19 + j] = y[j];
+        for (int j = 0; j < 19; ++j) ck[((int64_t)sgi * 19 + j) * C] = y[j];
         const int m = min(K, A.n_steps - sgi * K);
         for (int st = 0; st < m; ++st) rk4_step(A.G, y);
     }
-    for (int j = 0; j < 19; ++j) ck[nseg * 19 + j] = y[j];
+    for (int j = 0; j < 19; ++j) ck[((int64_t)nseg * 19 + j) * C] = y[j];
 
     // maps (.cu:566-582) in FP64, and dL/dy at the end from dL/dmaps
     const double th[3] = {z * cp, z * sp, -s};
@@ -514,7 +537,6 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
         dl2 += (u[i] - r[i]) * p2[i];
     }
     if (A.maps64) {
-        const int64_t C = A.count;
         A.maps64[0 * C + ray] = y[12];
         A.maps64[1 * C + ray] = y[16];
         A.maps64[2 * C + ray] = y[17];
@@ -523,15 +545,15 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
         A.maps64[5 * C + ray] = dl2;
     }
     if (!A.map_grad) return;
-    double lam[19];
-    for (int j = 0; j < 19; ++j) lam[j] = 0.0;
-    const int64_t C = A.count;
+    const int l = threadIdx.x;
+    double(*const vec)[19][kAdjBlock] = adj_vec;
+    for (int j = 0; j < 19; ++j) vec[kLam][j][l] = 0.0;
     const double mb[6] = {A.map_grad[0 * C + ray], A.map_grad[1 * C + ray], A.map_grad[2 * C + ray],
                           A.map_grad[3 * C + ray], A.map_grad[4 * C + ray], A.map_grad[5 * C + ray]};
-    lam[12] = mb[0];
-    lam[16] = mb[1];
-    lam[17] = mb[2];
-    lam[18] = mb[3];
+    vec[kLam][12][l] = mb[0];
+    vec[kLam][16][l] = mb[1];
+    vec[kLam][17][l] = mb[2];
+    vec[kLam][18][l] = mb[3];
     {
         // d(defl)/dx = (1 - u u^T) / |d| applied to mb4 theta-hat + mb5 p2
         double w[3], uw = 0;
@@ -539,54 +561,62 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
             w[i] = mb[4] * th[i] + mb[5] * p2[i];
             uw += u[i] * w[i];
         }
-        for (int i = 0; i < 3; ++i) lam[i] = (w[i] - u[i] * uw) / dn;
+        for (int i = 0; i < 3; ++i) vec[kLam][i][l] = (w[i] - u[i] * uw) / dn;
     }
 
-    double* wk = A.work + ray * (int64_t)(K + 4) * 19;  // K step states, then 4 stage states
-    double* stg = wk + K * 19;
+    // K step states, then 4 stage states, component-major like the checkpoints
+    double* const wk = A.work + ray;
+    auto W = [&](int q, int j) -> double& { return wk[((int64_t)q * 19 + j) * C]; };
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-        for (int j = 0; j < 19; ++j) y[j] = ck[sgi * 19 + j];
+        for (int j = 0; j < 19; ++j) y[j] = ck[((int64_t)sgi * 19 + j) * C];
         const int m = min(K, A.n_steps - sgi * K);
         for (int st = 0; st < m; ++st) {
-            for (int j = 0; j < 19; ++j) wk[st * 19 + j] = y[j];
+            for (int j = 0; j < 19; ++j) W(st, j) = y[j];
             rk4_step(A.G, y);
         }
         for (int st = m - 1; st >= 0; --st) {
-            // stage states of this step
+            // stage states of this step: s1 = y, s2 = y + k1/2, s3 = y + k2/2, s4 = y + k3
             {
-                double yy[19], k[19];
-                for (int j = 0; j < 19; ++j) yy[j] = wk[st * 19 + j];
-                for (int j = 0; j < 19; ++j) stg[j] = yy[j];
+                double yy[19], k[19], t[19];
+                for (int j = 0; j < 19; ++j) {
+                    yy[j] = W(st, j);
+                    W(K, j) = yy[j];
+                }
                 rhs19(A.G, yy, k);
-                for (int j = 0; j < 19; ++j) stg[19 + j] = yy[j] + 0.5 * k[j];
-                rhs19(A.G, stg + 19, k);
-                for (int j = 0; j < 19; ++j) stg[38 + j] = yy[j] + 0.5 * k[j];
-                rhs19(A.G, stg + 38, k);
-                for (int j = 0; j < 19; ++j) stg[57 + j] = yy[j] + k[j];
+                for (int j = 0; j < 19; ++j) {
+                    t[j] = yy[j] + 0.5 * k[j];
+                    W(K + 1, j) = t[j];
+                }
+                rhs19(A.G, t, k);
+                for (int j = 0; j < 19; ++j) {
+                    t[j] = yy[j] + 0.5 * k[j];
+                    W(K + 2, j) = t[j];
+                }
+                rhs19(A.G, t, k);
+                for (int j = 0; j < 19; ++j) W(K + 3, j) = yy[j] + k[j];
             }
             // reverse sweep: y' = y + (k1 + 2 k2 + 2 k3 + k4) / 6
-            double ybar[19], kb[19], sb[19];
             for (int j = 0; j < 19; ++j) {
-                ybar[j] = lam[j];
-                kb[j] = lam[j] * (1.0 / 6.0);
+                vec[kYbar][j][l] = vec[kLam][j][l];
+                vec[kKbar][j][l] = vec[kLam][j][l] * (1.0 / 6.0);
             }
-            stage_vjp(A, stg + 57, kb, sb);  // s4 = y + k3
+            stage_vjp(A, &W(K + 3, 0), C);  // s4 = y + k3
             for (int j = 0; j < 19; ++j) {
-                ybar[j] += sb[j];
-                kb[j] = sb[j] + lam[j] * (1.0 / 3.0);
+                vec[kYbar][j][l] += vec[kSbar][j][l];
+                vec[kKbar][j][l] = vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 3.0);
             }
-            stage_vjp(A, stg + 38, kb, sb);  // s3 = y + k2 / 2
+            stage_vjp(A, &W(K + 2, 0), C);  // s3 = y + k2 / 2
             for (int j = 0; j < 19; ++j) {
-                ybar[j] += sb[j];
-                kb[j] = 0.5 * sb[j] + lam[j] * (1.0 / 3.0);
+                vec[kYbar][j][l] += vec[kSbar][j][l];
+                vec[kKbar][j][l] = 0.5 * vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 3.0);
             }
-            stage_vjp(A, stg + 19, kb, sb);  // s2 = y + k1 / 2
+            stage_vjp(A, &W(K + 1, 0), C);  // s2 = y + k1 / 2
             for (int j = 0; j < 19; ++j) {
-                ybar[j] += sb[j];
-                kb[j] = 0.5 * sb[j] + lam[j] * (1.0 / 6.0);
+                vec[kYbar][j][l] += vec[kSbar][j][l];
+                vec[kKbar][j][l] = 0.5 * vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 6.0);
             }
-            stage_vjp(A, stg, kb, sb);  // s1 = y
-            for (int j = 0; j < 19; ++j) lam[j] = ybar[j] + sb[j];
+            stage_vjp(A, &W(K, 0), C);  // s1 = y
+            for (int j = 0; j < 19; ++j) vec[kLam][j][l] = vec[kYbar][j][l] + vec[kSbar][j][l];
         }
     }
 }
