@@ -104,16 +104,22 @@ __device__ __forceinline__ int h6(int i, int j) {
 }
 
 // Adds the gradient of  sum_o bar[o] * interp_o  with respect to the 64 tap
-// values (o = value, gradient, Hessian) into the padded gradient grid, with
-// one FP64 atomic per tap.  When every lane of the wavefront sits in the same
-// stencil (the rays near the observer), the 64 contributions are first summed
-// across the wavefront and one lane issues the atomics.
-__device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, size_t off, size_t py, size_t pz,
-                                             const Axis4& ax, const Axis4& ay, const Axis4& az, const double bar[10]) {
-    const bool uniform = __builtin_amdgcn_readfirstlane((int)off) == (int)off &&
-                         __builtin_amdgcn_readfirstlane((int)(off >> 32)) == (int)(off >> 32);
-    const bool all_uniform = __all(uniform);
-    // z-pass transpose: per slice c, adjoints of the six y-pass sums
+// values (o = value, gradient, Hessian) into the padded gradient grid.  The
+// workgroup is one wavefront.  Its lanes' stencils usually overlap (rays of
+// one sky patch), so the 64 x 64 tap contributions are first added in LDS over
+// the bounding box of the wavefront's stencils, and each touched cell then
+// takes one global FP64 atomic; a box larger than the tile falls back to one
+// global atomic per lane and tap.  Cell (x, y, z) of the padded grid is
+// x + py y + pz z; (cx, cy, cz) is the stencil's tap (-1, -1, -1).
+constexpr int kGradTileCells = 1024;
+__shared__ double grad_tile[kGradTileCells];
+__shared__ int grad_box[6];  // min x, y, z; max x, y, z of the stencil origins
+
+template <class F>
+__device__ __forceinline__ void for_each_tap(const Axis4& ax, const Axis4& ay, const Axis4& az, const double bar[10],
+                                             F&& f) {
+    // z-pass transpose: per slice c, adjoints of the six y-pass sums; the slices
+    // are a loop (not unrolled), which keeps the reverse sweep's code small
 #pragma unroll 1
     for (int c = 0; c < 4; ++c) {
         const double yv = az.w[0][c] * bar[0] + az.w[1][c] * bar[3] + az.w[2][c] * bar[9];
@@ -126,17 +132,58 @@ __device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, siz
             const double rv = ay.w[0][b] * yv + ay.w[1][b] * yd + ay.w[2][b] * ydd;
             const double rd = ay.w[0][b] * xd + ay.w[1][b] * xdyd;
             const double rdd = ay.w[0][b] * xdd;
-            for (int a = 0; a < 4; ++a) {
-                double pb = ax.w[0][a] * rv + ax.w[1][a] * rd + ax.w[2][a] * rdd;
-                const gptr<double> dst = (gptr<double>)(gbase_dev + off + a + b * py + c * pz);
-                if (all_uniform) {
-                    for (int m = 32; m >= 1; m >>= 1) pb += __shfl_xor(pb, m);
-                    if ((threadIdx.x & 63) == 0) (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, pb);
-                } else {
-                    (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, pb);
-                }
+            for (int a = 0; a < 4; ++a) f(a, b, c, ax.w[0][a] * rv + ax.w[1][a] * rd + ax.w[2][a] * rdd);
+        }
+    }
+}
+
+__device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, int cx, int cy, int cz, size_t py,
+                                             size_t pz, const Axis4& ax, const Axis4& ay, const Axis4& az,
+                                             const double bar[10]) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t active = __ballot(1);
+    const int n_active = __popcll(active);
+    const int rank = __popcll(active & ((uint64_t(1) << lane) - 1));
+    if (rank == 0) {
+        grad_box[0] = grad_box[1] = grad_box[2] = 0x7fffffff;
+        grad_box[3] = grad_box[4] = grad_box[5] = -1;
+    }
+    __syncthreads();
+    (void)__hip_atomic_fetch_min(&grad_box[0], cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    (void)__hip_atomic_fetch_min(&grad_box[1], cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    (void)__hip_atomic_fetch_min(&grad_box[2], cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    (void)__hip_atomic_fetch_max(&grad_box[3], cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    (void)__hip_atomic_fetch_max(&grad_box[4], cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    (void)__hip_atomic_fetch_max(&grad_box[5], cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    const int x0 = grad_box[0], y0 = grad_box[1], z0 = grad_box[2];
+    const int nx = grad_box[3] - x0 + 4, ny = grad_box[4] - y0 + 4, nz = grad_box[5] - z0 + 4;
+    __syncthreads();  // the box is read before the next call resets it
+    const int cells = nx * ny * nz;  // the same in every lane
+    if (cells <= kGradTileCells) {
+        for (int q = rank; q < cells; q += n_active) grad_tile[q] = 0.0;
+        __syncthreads();
+        const int t0 = (cx - x0) + nx * ((cy - y0) + ny * (cz - z0));
+        for_each_tap(ax, ay, az, bar, [&](int a, int b, int c, double pb) {
+            (void)__hip_atomic_fetch_add(&grad_tile[t0 + a + nx * (b + ny * c)], pb, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+        });
+        __syncthreads();
+        for (int q = rank; q < cells; q += n_active) {
+            const double v = grad_tile[q];
+            if (v != 0.0) {
+                const int qx = q % nx, qy = (q / nx) % ny, qz = q / (nx * ny);
+                const gptr<double> dst = (gptr<double>)(gbase_dev + (size_t)(x0 + qx) + (size_t)(y0 + qy) * py +
+                                                        (size_t)(z0 + qz) * pz);
+                (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, v);
             }
         }
+        __syncthreads();  // the tile is read before the next call clears it
+    } else {
+        const size_t off = (size_t)cx + (size_t)cy * py + (size_t)cz * pz;
+        for_each_tap(ax, ay, az, bar, [&](int a, int b, int c, double pb) {
+            (void)__builtin_amdgcn_global_atomic_fadd_f64((gptr<double>)(gbase_dev + off + a + b * py + c * pz), pb);
+        });
     }
 }
 
@@ -159,6 +206,7 @@ struct RhsTape {
 // they are used, so that four stencils do not hold 4 x 48 weights live.
 struct Stencil {
     size_t off;
+    int c[3];  // padded x, y, z index of tap (-1, -1, -1)
     double t[3];
 };
 
@@ -182,6 +230,9 @@ __device__ __forceinline__ void locate_stencils(const Grids& G, const double x[3
         const int iy = (a == 1) ? i1[1] : i0[1];
         const int iz = (a == 2) ? i1[2] : i0[2];
         st[f].off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
+        st[f].c[0] = ix;
+        st[f].c[1] = iy;
+        st[f].c[2] = iz;
         st[f].t[0] = a == 0 ? t1[0] : t0[0];
         st[f].t[1] = a == 1 ? t1[1] : t0[1];
         st[f].t[2] = a == 2 ? t1[2] : t0[2];
@@ -532,7 +583,7 @@ __device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], con
             for (int u = 0; u < 6; ++u) acc += bar[4 + u] * r3.t[t3idx(hu[u][0], hu[u][1], k)];
             xb[k] += acc;
         }
-        scatter_taps(gdst[f], st[f].off, py, pz, ax, ay, az, bar);
+        scatter_taps(gdst[f], st[f].c[0], st[f].c[1], st[f].c[2], py, pz, ax, ay, az, bar);
     }
 
     // normalisations: n = nd / |nd| etc.

@@ -431,6 +431,7 @@ struct AdjArgs {
     double* ckpt;            // count x (nseg + 1) x 19
     double* work;            // count x (K + 4) x 19
     double* grad[4];         // padded gradient grids: potential, B_x, B_y, B_z
+    const uint32_t* lane_slot;  // sorted lane -> slot map (srt::order_lanes), or null
 };
 
 // FP64 right-hand side on a 19-vector (production kernel arithmetic)
@@ -495,13 +496,18 @@ __device__ __noinline__ void stage_vjp(const AdjArgs& A, const double* __restric
 }
 
 __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
-    const int64_t ray = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
-    if (ray >= A.count) return;
-    const int64_t pix = A.pix_list ? A.pix_list[ray] : A.pix_begin + ray;
+    // lanes walk the rays in the order of A.lane_slot (compact sky patches, so
+    // a wavefront's stencils share cells and the gradient scatter stays in a
+    // small LDS tile); `lane` indexes the per-lane work buffers, `slot` the
+    // caller's arrays
+    const int64_t lane = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (lane >= A.count) return;
+    const int64_t slot = A.lane_slot ? (int64_t)A.lane_slot[lane] : lane;
+    const int64_t pix = A.pix_list ? A.pix_list[slot] : A.pix_begin + slot;
     const int64_t npix = 12 * A.nside * A.nside;
     if (pix < 0 || pix >= npix) {
         if (A.maps64)
-            for (int m = 0; m < 6; ++m) A.maps64[m * A.count + ray] = __builtin_nan("");
+            for (int m = 0; m < 6; ++m) A.maps64[m * A.count + slot] = __builtin_nan("");
         return;
     }
     double z, s, ph;
@@ -513,10 +519,10 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
     const int K = A.K;
     const int nseg = (A.n_steps + K - 1) / K;
     const int64_t C = A.count;
-    // checkpoints and segment states are component-major with the ray fastest
-    // (state q, component j of this ray at [(q * 19 + j) * C + ray]), so the
+    // checkpoints and segment states are component-major with the lane fastest
+    // (state q, component j of this lane at [(q * 19 + j) * C + lane]), so the
     // lanes of a wavefront read and write them coalesced
-    double* const ck = A.ckpt + ray;
+    double* const ck = A.ckpt + lane;
     for (int sgi = 0; sgi < nseg; ++sgi) {
         for (int j = 0; j < 19; ++j) ck[((int64_t)sgi * 19 + j) * C] = y[j];
         const int m = min(K, A.n_steps - sgi * K);
@@ -537,19 +543,19 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
         dl2 += (u[i] - r[i]) * p2[i];
     }
     if (A.maps64) {
-        A.maps64[0 * C + ray] = y[12];
-        A.maps64[1 * C + ray] = y[16];
-        A.maps64[2 * C + ray] = y[17];
-        A.maps64[3 * C + ray] = y[18];
-        A.maps64[4 * C + ray] = dl1;
-        A.maps64[5 * C + ray] = dl2;
+        A.maps64[0 * C + slot] = y[12];
+        A.maps64[1 * C + slot] = y[16];
+        A.maps64[2 * C + slot] = y[17];
+        A.maps64[3 * C + slot] = y[18];
+        A.maps64[4 * C + slot] = dl1;
+        A.maps64[5 * C + slot] = dl2;
     }
     if (!A.map_grad) return;
     const int l = threadIdx.x;
     double(*const vec)[19][kAdjBlock] = adj_vec;
     for (int j = 0; j < 19; ++j) vec[kLam][j][l] = 0.0;
-    const double mb[6] = {A.map_grad[0 * C + ray], A.map_grad[1 * C + ray], A.map_grad[2 * C + ray],
-                          A.map_grad[3 * C + ray], A.map_grad[4 * C + ray], A.map_grad[5 * C + ray]};
+    const double mb[6] = {A.map_grad[0 * C + slot], A.map_grad[1 * C + slot], A.map_grad[2 * C + slot],
+                          A.map_grad[3 * C + slot], A.map_grad[4 * C + slot], A.map_grad[5 * C + slot]};
     vec[kLam][12][l] = mb[0];
     vec[kLam][16][l] = mb[1];
     vec[kLam][17][l] = mb[2];
@@ -565,7 +571,7 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
     }
 
     // K step states, then 4 stage states, component-major like the checkpoints
-    double* const wk = A.work + ray;
+    double* const wk = A.work + lane;
     auto W = [&](int q, int j) -> double& { return wk[((int64_t)q * 19 + j) * C]; };
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         for (int j = 0; j < 19; ++j) y[j] = ck[((int64_t)sgi * 19 + j) * C];
@@ -1089,11 +1095,32 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     for (int i = 0; i < 3; ++i) A.obs[i] = observer ? observer[i] : 0.5 * (double)ctx->n;
     A.map_grad = map_grad;
     A.maps64 = maps64;
+    A.lane_slot = nullptr;
     A.ckpt = ctx->adj_buf;
     A.work = ctx->adj_buf + n_ck;
     for (int f = 0; f < 4; ++f) A.grad[f] = map_grad ? ctx->adj_buf + n_ck + n_wk + f * pvox : nullptr;
     if (map_grad) SRT_HIP(hipMemsetAsync(A.grad[0], 0, 4 * pvox * sizeof(double), st));
+    // rays in NESTED order of their pixels (srt::order_lanes); the results do
+    // not depend on it beyond the summation order of the gradient atomics
+    size_t map_bytes = 0;
+    if (count > 0 && count <= srt::kMaxOrderedRays) {
+        map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
+        const size_t order_work = srt::order_work_bytes(nside, count);
+        if (!order_work) return fail(SRT_ERR_DEVICE, "srt_trace_adjoint: lane-order work size");
+        if (map_bytes + order_work > ctx->order_bytes) {
+            SRT_HIP(hipDeviceSynchronize());  // an earlier launch may still read the old map
+            if (ctx->order_buf) SRT_HIP(hipFree(ctx->order_buf));
+            ctx->order_buf = nullptr;
+            ctx->order_bytes = 0;
+            SRT_HIP(ctx_malloc(ctx, &ctx->order_buf, map_bytes + order_work));
+            ctx->order_bytes = map_bytes + order_work;
+        }
+        A.lane_slot = static_cast<const uint32_t*>(ctx->order_buf);
+    }
     SRT_HIP(hipEventRecord(ctx->ev_start, st));
+    if (A.lane_slot)
+        SRT_HIP(srt::order_lanes(nside, pix_begin, count, pix_list, static_cast<uint32_t*>(ctx->order_buf),
+                                 static_cast<char*>(ctx->order_buf) + map_bytes, ctx->order_bytes - map_bytes, st));
     if (count > 0) {
         const int64_t blocks = (count + kAdjBlock - 1) / kAdjBlock;
         hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, A);
