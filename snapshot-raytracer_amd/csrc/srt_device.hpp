@@ -542,6 +542,11 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
         const int iz = (a == 2) ? i1[2] : i0[2];
         if constexpr (F::kBicubic) {
             static_assert(sizeof(RI) == 4, "bicubic records are FP32");
+            // each field's 16 record loads and its evaluation stay together:
+            // left free, the scheduler hoists later fields' loads into earlier
+            // fields, spills, and the kernel runs 10% slower at 2 wavefronts
+            // per SIMD (the same barrier on the padded-cube path costs 1%)
+            __builtin_amdgcn_sched_barrier(0);
             const Powers<RI> px = powers<RI>(a == 0 ? t1[0] : t0[0]);
             const Powers<RI> py = powers<RI>(a == 1 ? t1[1] : t0[1]);
             const Axis<RI> az = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
