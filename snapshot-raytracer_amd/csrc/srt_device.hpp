@@ -652,9 +652,12 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
     dy[12] = -Dp;
     const RA gam = n[0] * g[0] + n[1] * g[1] + n[2] * g[2] +
                    RA(0.25) * (n[0] * (RA)N[0] + n[1] * (RA)N[1] + n[2] * (RA)N[2]);
-    RA nHn = RA(0);
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) nHn += n[i] * H[i][j] * n[j];
+    RA nHn;
+    {
+        RA u[3];
+        for (int i = 0; i < 3; ++i) u[i] = H[i][0] * n[0] + H[i][1] * n[1] + H[i][2] * n[2];
+        nHn = n[0] * u[0] + n[1] * u[1] + n[2] * u[2];
+    }
     const RA PH = nHn - (H[0][0] + H[1][1] + H[2][2]);
     RA d13 = RA(-4) * Dp * gam - PH * D + RA(0.5) * (RA)L * D;
     // shear source uses H + T/2 (.cu:484-492)
@@ -667,12 +670,16 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
                 Hm[j][i] = Hm[i][j];
             }
     }
-    RA S1H = RA(0), S2H = RA(0);
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            S1H += (e1[i] * e1[j] - e2[i] * e2[j]) * Hm[i][j];
-            S2H += (e1[i] * e2[j] + e1[j] * e2[i]) * Hm[i][j];
+    RA S1H, S2H;
+    {
+        RA u1[3], u2[3];
+        for (int i = 0; i < 3; ++i) {
+            u1[i] = Hm[i][0] * e1[0] + Hm[i][1] * e1[1] + Hm[i][2] * e1[2];
+            u2[i] = Hm[i][0] * e2[0] + Hm[i][1] * e2[1] + Hm[i][2] * e2[2];
         }
+        S1H = (e1[0] * u1[0] + e1[1] * u1[1] + e1[2] * u1[2]) - (e2[0] * u2[0] + e2[1] * u2[1] + e2[2] * u2[2]);
+        S2H = RA(2) * (e1[0] * u2[0] + e1[1] * u2[1] + e1[2] * u2[2]);
+    }
     const RA D2 = D * D;
     dy[14] = RA(4) * s1 * gam + S1H * D2;
     dy[15] = RA(4) * s2 * gam + S2H * D2;
