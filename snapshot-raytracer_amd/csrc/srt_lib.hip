@@ -197,6 +197,17 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
     RA acc_reg[F::kStateInLds ? 1 : 19];
     double* y_lds = lds_state + threadIdx.x;
     RA* acc_lds = reinterpret_cast<RA*>(lds_state + 19 * kBlock) + threadIdx.x;
+    // FP32 forms also keep an FP32 copy of the ten components the right-hand
+    // side reads in FP32 (e1, e2, sigma, eps), beside the state, so a stage
+    // state is one FP32 multiply-add from it instead of an FP64 one between
+    // two conversions
+    constexpr bool kYsF32 = sizeof(RA) == 4;
+    RA ysf_reg[kYsF32 && !F::kStateInLds ? 10 : 1];
+    RA* ysf_lds = acc_lds + 19 * kBlock;
+    auto ysf = [&](int j) -> RA& {
+        if constexpr (F::kStateInLds) return ysf_lds[j * kBlock];
+        else return ysf_reg[j];
+    };
     auto y = [&](int j) -> double& {
         if constexpr (F::kStateInLds) return y_lds[j * kBlock];
         else return y_reg[j];
@@ -210,6 +221,10 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
                                  0.0,      0.0,      -1.0,     0.0,      0.0,     0.0, 0.0,    0.0};
 #pragma unroll
         for (int j = 0; j < 19; ++j) y(j) = init[j];
+        if constexpr (kYsF32) {
+#pragma unroll
+            for (int j = 0; j < 10; ++j) ysf(j) = (RA)init[j < 6 ? 6 + j : 8 + j];
+        }
     }
     // RK4 (.cu:513-550).  Position and dA increments stay in FP64 in every
     // form: x' carries the transverse deflection and the FP64 dA decides the
@@ -230,10 +245,17 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
         }
         const double Ds = y(12) + c * kDi;
         const double Dps = y(13) + c * (double)ki[13];
+        if constexpr (kYsF32) {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) ys[j] = (RA)(y(6 + j) + c * (double)ki[6 + j]);
+            for (int j = 0; j < 6; ++j) ys[j] = ysf(j) + (RA)c * ki[6 + j];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) ys[6 + j] = (RA)(y(14 + j) + c * (double)ki[14 + j]);
+            for (int j = 0; j < 4; ++j) ys[6 + j] = ysf(6 + j) + (RA)c * ki[14 + j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ys[j] = (RA)(y(6 + j) + c * (double)ki[6 + j]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ys[6 + j] = (RA)(y(14 + j) + c * (double)ki[14 + j]);
+        }
         srt::rhs<F>(G, xs, ns, Ds, Dps, ys, kxo, ko);
         kDo = -Dps;
     };
@@ -293,7 +315,14 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
         }
 #pragma unroll
         for (int j = 3; j < 19; ++j)
-            if (j != 12) y(j) += (double)(acc(j) + k[j]) * sixth;
+            if (j != 12) {
+                const double t = y(j) + (double)(acc(j) + k[j]) * sixth;
+                y(j) = t;
+                if constexpr (kYsF32) {
+                    if (j >= 6 && j < 12) ysf(j - 6) = (RA)t;
+                    if (j >= 14 && j < 18) ysf(j - 8) = (RA)t;
+                }
+            }
     }
     double yf[19];
 #pragma unroll
@@ -653,7 +682,10 @@ __global__ __launch_bounds__(256) void fold_grad_kernel(const double* __restrict
 // LDS bytes per block of trace_kernel<F>
 template <class F>
 constexpr unsigned lds_bytes() {
-    return F::kStateInLds ? (unsigned)(kBlock * 19 * (sizeof(double) + sizeof(typename F::RA))) : 0u;
+    using RA = typename F::RA;
+    if (!F::kStateInLds) return 0u;
+    // y (FP64) and the stage sum (RA), plus the FP32 copy of ten components
+    return (unsigned)(kBlock * (19 * (sizeof(double) + sizeof(RA)) + (sizeof(RA) == 4 ? 10 * sizeof(RA) : 0)));
 }
 
 }  // namespace

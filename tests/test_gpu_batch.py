@@ -132,3 +132,48 @@ def test_plan_errors(srt, instances):
     with pytest.raises(srt.SrtError):
         plan.launch(graph=True)  # graph replay needs a non-default stream
     plan.close()
+
+
+@pytest.mark.parametrize("tile,blocks", [(64, 1), (64, 5), (64, 512), (128, 37), (256, 2048)])
+def test_plan_many_tiles_per_workgroup(srt, fields, tile, blocks):
+    """One full-sky item and the 12 RING batches of a second sky as 13 items of
+    one plan: thousands of 64-lane groups drawn from each workgroup's tile slot
+    (one workgroup draws them all with blocks=1).  Every ray is traced exactly
+    once: outputs start as NaN and must equal srt_trace's bit for bit."""
+    import torch
+    print(f"[gpu] batch plan many tiles: tile {tile}, blocks {blocks}", flush=True)
+    dev = torch.device("cuda:0")
+    phi, shift = fields.snapshot(32, 77)
+    tr = srt.Tracer(0)
+    tr.upload(phi.to(dev), fields.interleave_shift(shift).to(dev))
+    nside, n_steps = 16, 6
+    rays = 12 * nside * nside
+    full = torch.empty(6 * rays, dtype=torch.float32, device=dev)
+    tr.trace_device(nside, n_steps, full, precision=srt.PREC_MIXED)
+    torch.cuda.synchronize()
+    ref_full = full.cpu().numpy()
+    items = [dict(tracer=tr, nside=nside, pix_begin=0, count=rays, n_steps=n_steps,
+                  out=torch.full((6 * rays,), float("nan"), dtype=torch.float32, device=dev))]
+    per = nside * nside
+    ref_b = []
+    for b in range(12):
+        o = torch.empty(6 * per, dtype=torch.float32, device=dev)
+        tr.trace_device(nside, n_steps + 1, o, pix_begin=b * per, count=per, precision=srt.PREC_MIXED)
+        ref_b.append(o)
+        items.append(dict(tracer=tr, nside=nside, pix_begin=b * per, count=per, n_steps=n_steps + 1,
+                          out=torch.full((6 * per,), float("nan"), dtype=torch.float32, device=dev)))
+    torch.cuda.synchronize()
+    plan = srt.Plan(items, precision=srt.PREC_MIXED)
+    plan.configure(tile, blocks)
+    s = torch.cuda.Stream()
+    for rep in range(3):
+        for it in items:
+            it["out"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            plan.launch(stream=s, graph=rep > 0)
+        s.synchronize()
+        assert np.array_equal(items[0]["out"].cpu().numpy(), ref_full)
+        for it, r in zip(items[1:], ref_b):
+            assert np.array_equal(it["out"].cpu().numpy(), r.cpu().numpy())
+    plan.close()
