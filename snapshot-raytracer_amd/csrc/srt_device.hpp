@@ -262,9 +262,13 @@ __device__ __forceinline__ Powers<R> powers(R t) {
 }
 
 // Wraps floor(x) into [0, n) and returns the offset x - floor(x) (exact in FP64).
+// kPow2: n is a power of two, and the wrap is the low bits of floor(x) (two's
+// complement, so also for negative x).
+template <bool kPow2 = false>
 __device__ __forceinline__ int locate(double x, int n, double inv_n, double& t) {
     const double fl = floor(x);
     t = x - fl;
+    if constexpr (kPow2) return (int)fl & (n - 1);
     const double q = floor(fl * inv_n);
     // fl and q * n are integers below 2^53, so r is exact; one correction
     // covers the rounding of fl * inv_n
@@ -437,6 +441,13 @@ struct Form {
     static constexpr bool kGrid64 = kGrid64_;
     static constexpr bool kBicubic = kBicubic_;
     static constexpr int kWavesPerSimd = kWavesPerSimd_;
+    static constexpr bool kPow2 = false;  // grid edge known to be a power of two
+};
+// A form on a grid whose edge is a power of two (the periodic wrap of a cell
+// index is a mask: ~2 % of the mixed kernels' time at 512^3).
+template <class F>
+struct Pow2Grid : F {
+    static constexpr bool kPow2 = true;
 };
 using FormFP64 = Form<double, double, false, true, true>;
 using FormMixed = Form<float, float, false, false, false>;
@@ -510,7 +521,7 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         double t;
-        i0[a] = locate(x[a], G.n, G.inv_n, t);
+        i0[a] = locate<F::kPow2>(x[a], G.n, G.inv_n, t);
         t0[a] = cell_offset<F>(x[a], t);
         if constexpr (sizeof(RI) == 8) {
             // the staggered cell follows from the unstaggered one: floor(x - 1/2)
@@ -525,7 +536,7 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
             // derivation above (measured 5% faster for the full kernel)
             const double xs = x[a] - 0.5;
             double ts;
-            i1[a] = locate(xs, G.n, G.inv_n, ts);
+            i1[a] = locate<F::kPow2>(xs, G.n, G.inv_n, ts);
             t1[a] = cell_offset<F>(xs, ts);
         }
     }

@@ -688,6 +688,15 @@ constexpr unsigned lds_bytes() {
     return (unsigned)(kBlock * (19 * (sizeof(double) + sizeof(RA)) + (sizeof(RA) == 4 ? 10 * sizeof(RA) : 0)));
 }
 
+// trace_kernel<F> on `blocks` workgroups, specialised for a power-of-two grid edge
+template <class F>
+void launch_trace(bool pow2, unsigned blocks, hipStream_t st, const TraceArgs& A) {
+    if (pow2)
+        hipLaunchKernelGGL(trace_kernel<srt::Pow2Grid<F>>, dim3(blocks), dim3(kBlock), lds_bytes<F>(), st, A);
+    else
+        hipLaunchKernelGGL(trace_kernel<F>, dim3(blocks), dim3(kBlock), lds_bytes<F>(), st, A);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI ----
@@ -1047,23 +1056,20 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
                                  static_cast<char*>(ctx->order_buf) + map_bytes, ctx->order_bytes - map_bytes, st));
     }
     if (count > 0) {
-        const int64_t blocks = (count + kBlock - 1) / kBlock;
+        const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+        const bool pow2 = (ctx->n & (ctx->n - 1)) == 0;
         switch (precision) {
             case SRT_PREC_FP64:
-                hipLaunchKernelGGL(trace_kernel<srt::FormFP64>, dim3((unsigned)blocks), dim3(kBlock),
-                                   lds_bytes<srt::FormFP64>(), st, A);
+                launch_trace<srt::FormFP64>(pow2, blocks, st, A);
                 break;
             case SRT_PREC_MIXED:
                 if (ctx->rec)
-                    hipLaunchKernelGGL(trace_kernel<srt::FormMixedBicubic>, dim3((unsigned)blocks), dim3(kBlock),
-                                       lds_bytes<srt::FormMixedBicubic>(), st, A);
+                    launch_trace<srt::FormMixedBicubic>(pow2, blocks, st, A);
                 else
-                    hipLaunchKernelGGL(trace_kernel<srt::FormMixed>, dim3((unsigned)blocks), dim3(kBlock),
-                                       lds_bytes<srt::FormMixed>(), st, A);
+                    launch_trace<srt::FormMixed>(pow2, blocks, st, A);
                 break;
             case SRT_PREC_REF:
-                hipLaunchKernelGGL(trace_kernel<srt::FormRef>, dim3((unsigned)blocks), dim3(kBlock),
-                                   lds_bytes<srt::FormRef>(), st, A);
+                launch_trace<srt::FormRef>(pow2, blocks, st, A);
                 break;
             default:
                 return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
@@ -1181,6 +1187,7 @@ struct srt_plan {
     int device = 0;
     int precision = SRT_PREC_MIXED;
     bool bicubic = false;  // mixed plan whose contexts all hold bicubic records
+    bool pow2 = false;     // every item's grid edge is a power of two
     int n_inst = 0;
     int64_t total_rays = 0, total_ray_steps = 0;
     std::vector<srt_batch_item> items;
@@ -1251,6 +1258,8 @@ int plan_build_table(srt_plan* p) {
     int64_t tiles = 0;
     p->bicubic = p->precision == SRT_PREC_MIXED;
     for (int i = 0; i < p->n_inst; ++i) p->bicubic = p->bicubic && p->items[i].ctx->rec != nullptr;
+    p->pow2 = true;
+    for (int i = 0; i < p->n_inst; ++i) p->pow2 = p->pow2 && (p->items[i].ctx->n & (p->items[i].ctx->n - 1)) == 0;
     for (int i = 0; i < p->n_inst; ++i) {
         const srt_batch_item& it = p->items[i];
         srt_context* c = it.ctx;
@@ -1285,26 +1294,31 @@ int plan_build_table(srt_plan* p) {
     return SRT_OK;
 }
 
+// batch_kernel<F> for plan p, specialised when every item's grid edge is a power of two
+template <class F>
+void launch_batch(const srt_plan* p, hipStream_t st) {
+    if (p->pow2)
+        hipLaunchKernelGGL(batch_kernel<srt::Pow2Grid<F>>, dim3(p->blocks), dim3(kBlock), lds_bytes<F>(), st,
+                           p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->groups, p->d_counter);
+    else
+        hipLaunchKernelGGL(batch_kernel<F>, dim3(p->blocks), dim3(kBlock), lds_bytes<F>(), st, p->d_inst, p->n_inst,
+                           p->total_tiles, p->tile_rays, p->groups, p->d_counter);
+}
+
 int plan_enqueue(srt_plan* p, hipStream_t st) {
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, p->d_counter, p->groups);
     switch (p->precision) {
         case SRT_PREC_FP64:
-            hipLaunchKernelGGL(batch_kernel<srt::FormFP64>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormFP64>(),
-                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->groups, p->d_counter);
+            launch_batch<srt::FormFP64>(p, st);
             break;
         case SRT_PREC_MIXED:
             if (p->bicubic)
-                hipLaunchKernelGGL(batch_kernel<srt::FormMixedBicubic>, dim3(p->blocks), dim3(kBlock),
-                                   lds_bytes<srt::FormMixedBicubic>(), st, p->d_inst, p->n_inst, p->total_tiles,
-                                   p->tile_rays, p->groups, p->d_counter);
+                launch_batch<srt::FormMixedBicubic>(p, st);
             else
-                hipLaunchKernelGGL(batch_kernel<srt::FormMixed>, dim3(p->blocks), dim3(kBlock),
-                                   lds_bytes<srt::FormMixed>(), st, p->d_inst, p->n_inst, p->total_tiles,
-                                   p->tile_rays, p->groups, p->d_counter);
+                launch_batch<srt::FormMixed>(p, st);
             break;
         default:
-            hipLaunchKernelGGL(batch_kernel<srt::FormRef>, dim3(p->blocks), dim3(kBlock), lds_bytes<srt::FormRef>(),
-                               st, p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->groups, p->d_counter);
+            launch_batch<srt::FormRef>(p, st);
             break;
     }
     SRT_HIP(hipGetLastError());
