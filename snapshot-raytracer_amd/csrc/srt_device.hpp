@@ -597,6 +597,11 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
     RI T[6] = {RI(0), RI(0), RI(0), RI(0), RI(0), RI(0)};  // xx xy xz yy yz zz
     RI L = RI(0), E1 = RI(0), E2 = RI(0);
     RA Bv[3];
+    // FP32 forms: with G_aj = dB_a/dx_j, E = 1/2 e . (G n + G^T n) and G^T n = N,
+    // so only G n is needed beside N (the FP64 and REF forms keep the
+    // reference's per-component order and rounding points)
+    constexpr bool kFusedE = sizeof(RA) == 4;
+    RA Gn[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const Interp<RI> r = interp(1 + a);
@@ -606,11 +611,15 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
                              {(RA)r.hxy, (RA)r.hyy, (RA)r.hyz},
                              {(RA)r.hxz, (RA)r.hyz, (RA)r.hzz}};
         const RA ng = n[0] * gb[0] + n[1] * gb[1] + n[2] * gb[2];
-        const RA e1g = e1[0] * gb[0] + e1[1] * gb[1] + e1[2] * gb[2];
-        const RA e2g = e2[0] * gb[0] + e2[1] * gb[1] + e2[2] * gb[2];
         for (int j = 0; j < 3; ++j) N[j] = (RI)((RA)N[j] + n[a] * gb[j]);
-        E1 = (RI)((RA)E1 + RA(0.5) * (e1[a] * ng + n[a] * e1g));
-        E2 = (RI)((RA)E2 + RA(0.5) * (e2[a] * ng + n[a] * e2g));
+        if constexpr (kFusedE) {
+            Gn[a] = ng;
+        } else {
+            const RA e1g = e1[0] * gb[0] + e1[1] * gb[1] + e1[2] * gb[2];
+            const RA e2g = e2[0] * gb[0] + e2[1] * gb[1] + e2[2] * gb[2];
+            E1 = (RI)((RA)E1 + RA(0.5) * (e1[a] * ng + n[a] * e1g));
+            E2 = (RI)((RA)E2 + RA(0.5) * (e2[a] * ng + n[a] * e2g));
+        }
         RA nH[3];
         for (int j = 0; j < 3; ++j) nH[j] = n[0] * Hb[0][j] + n[1] * Hb[1][j] + n[2] * Hb[2][j];
         int k = 0;
@@ -622,6 +631,11 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
                 T[k] = (RI)((RA)T[k] + c);
             }
         L = (RI)((RA)L + n[a] * (Hb[0][0] + Hb[1][1] + Hb[2][2]));
+    }
+    if constexpr (kFusedE) {
+        const RA s0 = Gn[0] + (RA)N[0], s1 = Gn[1] + (RA)N[1], s2 = Gn[2] + (RA)N[2];
+        E1 = RA(0.5) * (e1[0] * s0 + e1[1] * s1 + e1[2] * s2);
+        E2 = RA(0.5) * (e2[0] * s0 + e2[1] * s1 + e2[2] * s2);
     }
 
     // x' = -n (1 + 2 phi) - B, with the norm of n in FP64: the position decides
