@@ -1002,6 +1002,8 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: pixel range outside the sky");
     if (ray_order != SRT_ORDER_NESTED && ray_order != SRT_ORDER_LINEAR)
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown ray order");
+    if (precision != SRT_PREC_FP64 && precision != SRT_PREC_MIXED && precision != SRT_PREC_REF)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
     SRT_HIP(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     // NESTED over the full sky at power-of-two Nside: lanes are NESTED pixels;

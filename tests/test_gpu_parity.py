@@ -192,6 +192,12 @@ def test_empty_trace_and_errors(srt, tracer):
     with pytest.raises(srt.SrtError) as ei:
         tracer.trace_device(NSIDE, 4, None)
     assert ei.value.status == 1
+    import torch
+    o = torch.zeros(6 * 12 * NSIDE * NSIDE, dtype=torch.float32, device="cuda:0")
+    with pytest.raises(srt.SrtError) as ei:
+        tracer.trace_device(NSIDE, 4, o, precision=7)
+    assert ei.value.status == 1
+    assert float(o.abs().max()) == 0.0  # rejected before any work
 
 
 @pytest.mark.parametrize("n", [16, 20])
