@@ -178,7 +178,7 @@ __device__ __forceinline__ int64_t tile_of_block(int64_t b, int64_t nb, int g) {
 
 // Traces one ray (RING pixel `pix`) and writes its maps to slot `slot` of
 // the map-major output (and the final state if `state` is not null).
-template <class F>
+template <class F, int kLanes = kBlock>
 __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, int64_t pix, int64_t slot,
                                           int64_t count, int n_steps, const double* __restrict__ obs,
                                           float* __restrict__ out, double* __restrict__ state) {
@@ -196,24 +196,24 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
     double y_reg[F::kStateInLds ? 1 : 19];
     RA acc_reg[F::kStateInLds ? 1 : 19];
     double* y_lds = lds_state + threadIdx.x;
-    RA* acc_lds = reinterpret_cast<RA*>(lds_state + 19 * kBlock) + threadIdx.x;
+    RA* acc_lds = reinterpret_cast<RA*>(lds_state + 19 * kLanes) + threadIdx.x;
     // FP32 forms also keep an FP32 copy of the ten components the right-hand
     // side reads in FP32 (e1, e2, sigma, eps), beside the state, so a stage
     // state is one FP32 multiply-add from it instead of an FP64 one between
     // two conversions
     constexpr bool kYsF32 = sizeof(RA) == 4;
     RA ysf_reg[kYsF32 && !F::kStateInLds ? 10 : 1];
-    RA* ysf_lds = acc_lds + 19 * kBlock;
+    RA* ysf_lds = acc_lds + 19 * kLanes;
     auto ysf = [&](int j) -> RA& {
-        if constexpr (F::kStateInLds) return ysf_lds[j * kBlock];
+        if constexpr (F::kStateInLds) return ysf_lds[j * kLanes];
         else return ysf_reg[j];
     };
     auto y = [&](int j) -> double& {
-        if constexpr (F::kStateInLds) return y_lds[j * kBlock];
+        if constexpr (F::kStateInLds) return y_lds[j * kLanes];
         else return y_reg[j];
     };
     auto acc = [&](int j) -> RA& {
-        if constexpr (F::kStateInLds) return acc_lds[j * kBlock];
+        if constexpr (F::kStateInLds) return acc_lds[j * kLanes];
         else return acc_reg[j];
     };
     {
@@ -347,9 +347,19 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
     }
 }
 
+// Workgroup size of trace_kernel<F>: 128 lanes for the record kernel (1.7 %
+// faster than 256 at 512^3 / Nside 1024: a workgroup's LDS is free for the next
+// one as soon as its two wavefronts end), 256 for the other forms (which lose
+// 3-13 % at 128).  The persistent batch kernel keeps 256 for every form.
 template <class F>
-__global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void trace_kernel(TraceArgs A) {
-    const int64_t lane_ray = tile_of_block(blockIdx.x, gridDim.x, A.xcd_groups) * kBlock + threadIdx.x;
+constexpr int trace_lanes() {
+    return F::kBicubic ? 128 : kBlock;
+}
+
+template <class F>
+__global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void trace_kernel(TraceArgs A) {
+    constexpr int kLanes = trace_lanes<F>();
+    const int64_t lane_ray = tile_of_block(blockIdx.x, gridDim.x, A.xcd_groups) * kLanes + threadIdx.x;
     if (lane_ray >= A.count) return;
     int64_t pix, slot;
     const bool nested_sky = A.order == SRT_ORDER_NESTED && !A.lane_slot;
@@ -369,7 +379,7 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void trace_kernel(TraceAr
         }
         return;
     }
-    trace_ray<F>(A.G, A.nside, pix, slot, A.count, A.n_steps, A.obs, A.out, A.state);
+    trace_ray<F, kLanes>(A.G, A.nside, pix, slot, A.count, A.n_steps, A.obs, A.out, A.state);
 }
 
 // ---------------------------------------------------------------- batch ----
@@ -679,22 +689,25 @@ __global__ __launch_bounds__(256) void fold_grad_kernel(const double* __restrict
     }
 }
 
-// LDS bytes per block of trace_kernel<F>
-template <class F>
+// LDS bytes per workgroup of kLanes lanes tracing with form F
+template <class F, int kLanes = kBlock>
 constexpr unsigned lds_bytes() {
     using RA = typename F::RA;
     if (!F::kStateInLds) return 0u;
     // y (FP64) and the stage sum (RA), plus the FP32 copy of ten components
-    return (unsigned)(kBlock * (19 * (sizeof(double) + sizeof(RA)) + (sizeof(RA) == 4 ? 10 * sizeof(RA) : 0)));
+    return (unsigned)(kLanes * (19 * (sizeof(double) + sizeof(RA)) + (sizeof(RA) == 4 ? 10 * sizeof(RA) : 0)));
 }
 
-// trace_kernel<F> on `blocks` workgroups, specialised for a power-of-two grid edge
+// trace_kernel<F> over A.count lanes, specialised for a power-of-two grid edge
 template <class F>
-void launch_trace(bool pow2, unsigned blocks, hipStream_t st, const TraceArgs& A) {
+void launch_trace(bool pow2, hipStream_t st, const TraceArgs& A) {
+    constexpr int lanes = trace_lanes<F>();
+    constexpr unsigned lds = lds_bytes<F, lanes>();
+    const unsigned blocks = (unsigned)((A.count + lanes - 1) / lanes);
     if (pow2)
-        hipLaunchKernelGGL(trace_kernel<srt::Pow2Grid<F>>, dim3(blocks), dim3(kBlock), lds_bytes<F>(), st, A);
+        hipLaunchKernelGGL(trace_kernel<srt::Pow2Grid<F>>, dim3(blocks), dim3(lanes), lds, st, A);
     else
-        hipLaunchKernelGGL(trace_kernel<F>, dim3(blocks), dim3(kBlock), lds_bytes<F>(), st, A);
+        hipLaunchKernelGGL(trace_kernel<F>, dim3(blocks), dim3(lanes), lds, st, A);
 }
 
 }  // namespace
@@ -1058,20 +1071,19 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
                                  static_cast<char*>(ctx->order_buf) + map_bytes, ctx->order_bytes - map_bytes, st));
     }
     if (count > 0) {
-        const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
         const bool pow2 = (ctx->n & (ctx->n - 1)) == 0;
         switch (precision) {
             case SRT_PREC_FP64:
-                launch_trace<srt::FormFP64>(pow2, blocks, st, A);
+                launch_trace<srt::FormFP64>(pow2, st, A);
                 break;
             case SRT_PREC_MIXED:
                 if (ctx->rec)
-                    launch_trace<srt::FormMixedBicubic>(pow2, blocks, st, A);
+                    launch_trace<srt::FormMixedBicubic>(pow2, st, A);
                 else
-                    launch_trace<srt::FormMixed>(pow2, blocks, st, A);
+                    launch_trace<srt::FormMixed>(pow2, st, A);
                 break;
             case SRT_PREC_REF:
-                launch_trace<srt::FormRef>(pow2, blocks, st, A);
+                launch_trace<srt::FormRef>(pow2, st, A);
                 break;
             default:
                 return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
