@@ -347,13 +347,14 @@ __device__ __forceinline__ void trace_ray(const srt::Grids& G, int64_t nside, in
     }
 }
 
-// Workgroup size of trace_kernel<F>: 128 lanes for the record kernel (1.7 %
-// faster than 256 at 512^3 / Nside 1024: a workgroup's LDS is free for the next
-// one as soon as its two wavefronts end), 256 for the other forms (which lose
-// 3-13 % at 128).  The persistent batch kernel keeps 256 for every form.
+// Workgroup size of trace_kernel<F>: one wavefront for the record kernel (2 %
+// faster than 256 lanes at 512^3 / Nside 1024, 0.5 % faster than 128: a
+// workgroup's LDS is free for the next one as soon as its wavefront ends), 256
+// lanes for the other forms (which lose 3-13 % at 128).  The persistent batch
+// kernel keeps 256 for every form.
 template <class F>
 constexpr int trace_lanes() {
-    return F::kBicubic ? 128 : kBlock;
+    return F::kBicubic ? 64 : kBlock;
 }
 
 template <class F>
