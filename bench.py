@@ -32,7 +32,7 @@ import __graft_entry__  # noqa: E402
 FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12777.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
-HBM_BYTES_PER_LAUNCH = {"mixed": 27.42e9, "mixed_taps": 166.0e9, "fp64": 8.02e9}
+HBM_BYTES_PER_LAUNCH = {"mixed": 27.42e9, "mixed_taps": 168.1e9, "fp64": 8.03e9}
 # The reference algorithm's count (oracle with an operation-counting scalar).
 REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
 # MI355X published vector peaks (FP32 counts packed FP32; this build disables
@@ -58,7 +58,7 @@ def parse():
                    help="comma list of: mixed (FP32 interpolation from bicubic cell records), "
                         "mixed_taps (same arithmetic on the padded cubes), fp64, ref (the reference GPU "
                         "code's float/double mix, FP32 shift)")
-    p.add_argument("--brick", type=int, default=8, help="column brick edge of the bicubic records")
+    p.add_argument("--brick", type=int, default=16, help="column brick edge of the bicubic records")
     p.add_argument("--order", default="nested", choices=["nested", "linear"])
     p.add_argument("--subset-frac", type=float, default=0.01)
     p.add_argument("--no-oracle", action="store_true", help="skip the CPU oracle leg (accuracy + cpu baseline)")
@@ -98,7 +98,7 @@ def synth(fields, n, seed, dev):
 
 def workload_large(args, pkg, fields, dev):
     """1024^3 potential + 3 x 1024^3 shift, Nside 2048 full sky, 512 steps, mixed, one launch per
-    layout and lane order: bicubic cell records in 8x8 column bricks (also as the 12 RING batches in
+    layout and lane order: bicubic cell records in 16x16 column bricks (also as the 12 RING batches in
     one persistent plan launch) and in row-major columns (when they fit in device memory), then the
     padded grids."""
     import torch
@@ -129,8 +129,8 @@ def workload_large(args, pkg, fields, dev):
 
     both = (("nested", pkg.ORDER_NESTED), ("linear", pkg.ORDER_LINEAR))
     try:
-        tr.set_layout(pkg.LAYOUT_BICUBIC, 8)
-        run("bicubic_brick8", both)
+        tr.set_layout(pkg.LAYOUT_BICUBIC, 16)
+        run("bicubic_brick16", both)
         # the 12 RING batches of the sky as 12 items of one persistent launch (default configuration:
         # 64-ray tiles, 2 blocks per CU, NESTED lanes within each batch, 8 XCD tile ranges)
         per = nside * nside
@@ -139,9 +139,9 @@ def workload_large(args, pkg, fields, dev):
         plan = pkg.Plan(items, precision=pkg.PREC_MIXED)
         plan.launch(stream=stream)
         ms = plan.last_ms()
-        res["bicubic_brick8"]["plan_12_batches"] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0),
+        res["bicubic_brick16"]["plan_12_batches"] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0),
                                                     "config": plan.config()[:2]}
-        print(f"[bench] large bicubic_brick8, 12 batches in one persistent launch: {ms:.0f} ms",
+        print(f"[bench] large bicubic_brick16, 12 batches in one persistent launch: {ms:.0f} ms",
               file=sys.stderr, flush=True)
         plan.close()
         del items, plan
@@ -255,7 +255,7 @@ def workload_batch(args, pkg, fields, dev):
     for i, (n, nside, b, n_steps, obs) in enumerate(params):
         tr = pkg.Tracer(0)
         if layout == "bicubic":
-            tr.set_layout(pkg.LAYOUT_BICUBIC, 8)
+            tr.set_layout(pkg.LAYOUT_BICUBIC, 16)
         phi, shift_il, _ = synth(fields, n, 1000 + 4 * i, dev)
         tr.upload(phi, shift_il)
         del phi, shift_il
@@ -321,7 +321,7 @@ def workload_batch(args, pkg, fields, dev):
 
 def workload_layouts(args, pkg, fields, dev):
     """512^3 / Nside 1024 / 256 steps, mixed kernel: padded cubes vs bicubic records with column
-    bricks of edge 1 (row-major columns), 4, 8 and 16, each in NESTED and linear lane order."""
+    bricks of edge 1 (row-major columns), 4, 8, 16 and 32, each in NESTED and linear lane order."""
     import torch
     n, nside, n_steps = args.grid, args.nside, args.n_steps
     rays = 12 * nside * nside
@@ -334,7 +334,7 @@ def workload_layouts(args, pkg, fields, dev):
     out = torch.empty(6 * rays, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
     res = {}
-    for lay, brick in (("taps", 8), ("bicubic", 1), ("bicubic", 4), ("bicubic", 8), ("bicubic", 16)):
+    for lay, brick in (("taps", 8), ("bicubic", 1), ("bicubic", 4), ("bicubic", 8), ("bicubic", 16), ("bicubic", 32)):
         t0 = time.time()
         tr.set_layout(pkg.LAYOUT_TAPS if lay == "taps" else pkg.LAYOUT_BICUBIC, brick)
         torch.cuda.synchronize()

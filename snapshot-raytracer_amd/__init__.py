@@ -171,7 +171,7 @@ class Tracer:
                "srt_upload")
         self.n = nx
 
-    def set_layout(self, layout: int, brick: int = 8) -> None:
+    def set_layout(self, layout: int, brick: int = 16) -> None:
         """Device layout of the snapshot (LAYOUT_TAPS or LAYOUT_BICUBIC) and column brick edge.
 
         LAYOUT_BICUBIC adds per-cell bicubic coefficient records, which mixed

@@ -156,7 +156,7 @@ int main(int argc, char* argv[]) {
     std::vector<float>().swap(phi);
     std::vector<float>().swap(shift);
     if (precision == SRT_PREC_MIXED && layout_s != "taps") {
-        rc = srt_set_layout(ctx, SRT_LAYOUT_BICUBIC, 8);
+        rc = srt_set_layout(ctx, SRT_LAYOUT_BICUBIC, 16);
         if (rc == SRT_ERR_OUT_OF_MEMORY && layout_s == "auto") {
             std::cout << "Bicubic cell records do not fit in device memory; tracing from the padded grids."
                       << std::endl;

@@ -728,7 +728,7 @@ struct srt_context {
     double* grids64 = nullptr;
     // bicubic cell records of the four fields (SRT_LAYOUT_BICUBIC)
     int layout = SRT_LAYOUT_TAPS;
-    int brick_shift = 3;
+    int brick_shift = 4;  // 16 x 16 column bricks
     float* rec = nullptr;
     size_t rec_field_floats = 0;
     // workgroup -> tile grouping of trace launches (tile_of_block); from
