@@ -239,12 +239,34 @@ __device__ __forceinline__ void locate_stencils(const Grids& G, const double x[3
     }
 }
 
-// Right-hand side at stage state s (19, FP64), keeping the intermediates.
-// The interpolations read the FP64 copies of the grids (value, gradient,
-// Hessian); the third derivatives the position adjoint needs are
-// re-interpolated in rhs_vjp rather than kept live from here.
+// Value, gradient and Hessian of the four fields at their stencils (FP64 grids).
+__device__ __forceinline__ void interp_fields(const Grids& G, const Stencil st[4], Interp<double> q[4]) {
+    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
+    const double* grids[4] = {G.phi64, G.b64[0], G.b64[1], G.b64[2]};
+    for (int f = 0; f < 4; ++f) {
+        const Axis<double> wx = cr_axis<double>(st[f].t[0]), wy = cr_axis<double>(st[f].t[1]),
+                           wz = cr_axis<double>(st[f].t[2]);
+        q[f] = tricubic<double>(grids[f] + st[f].off, py, pz, wx, wy, wz);
+    }
+}
+
+// Right-hand side at stage state s (19, FP64), keeping the intermediates,
+// from the four fields' interpolants at s (interp_fields).  The third
+// derivatives the position adjoint needs are re-interpolated in rhs_vjp
+// rather than kept live from here.
+__device__ __forceinline__ void rhs_forward_from(const double* s, const Interp<double> qf[4], RhsTape& T,
+                                                 double dy[19]);
+
+// The same with the interpolations done here.
 __device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, const Stencil st[4], RhsTape& T,
                                             double dy[19]) {
+    Interp<double> q[4];
+    interp_fields(G, st, q);
+    rhs_forward_from(s, q, T, dy);
+}
+
+__device__ __forceinline__ void rhs_forward_from(const double* s, const Interp<double> qf[4], RhsTape& T,
+                                                 double dy[19]) {
     for (int i = 0; i < 3; ++i) {
         T.nd[i] = s[3 + i];
         T.e1d[i] = s[6 + i];
@@ -264,12 +286,8 @@ __device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, con
         T.e1[i] = T.e1d[i] * T.r1;
         T.e2[i] = T.e2d[i] * T.r2;
     }
-    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
-    const double* grids[4] = {G.phi64, G.b64[0], G.b64[1], G.b64[2]};
     for (int f = 0; f < 4; ++f) {
-        const Axis<double> wx = cr_axis<double>(st[f].t[0]), wy = cr_axis<double>(st[f].t[1]),
-                           wz = cr_axis<double>(st[f].t[2]);
-        const Interp<double> q = tricubic<double>(grids[f] + st[f].off, py, pz, wx, wy, wz);
+        const Interp<double>& q = qf[f];
         const struct {
             double v, g[3];
         } r = {q.v, {q.gx, q.gy, q.gz}};

@@ -469,7 +469,7 @@ struct AdjArgs {
     const double* map_grad;  // 6 x count, null = forward only
     double* maps64;          // 6 x count, may be null
     double* ckpt;            // count x (nseg + 1) x 19
-    double* work;            // count x (K + 4) x 19
+    double* work;            // count x ((K + 5) x 19 + 160): step and stage states, a stage RHS, interpolants
     double* grad[4];         // padded gradient grids: potential, B_x, B_y, B_z
     const uint32_t* lane_slot;  // sorted lane -> slot map (srt::order_lanes), or null
 };
@@ -510,11 +510,59 @@ __device__ __forceinline__ void rk4_step(const srt::Grids& G, double* y) {
 enum { kLam = 0, kYbar = 1, kKbar = 2, kSbar = 3 };
 __shared__ double adj_vec[4][19][kAdjBlock];
 
+// Per-lane buffers are component-major with the lane fastest: component j of
+// a lane's vector at p[j * stride].  The 40 interpolants of a stage (value,
+// gradient and Hessian of the four fields) are stored the same way, field f,
+// output o at component 10 f + o.
+constexpr int kInterpDoubles = 40;
+
+__device__ __forceinline__ void store_interps(double* q_out, int64_t stride, const srt::Interp<double> q[4]) {
+    const srt::adj::gptr<double> g = (srt::adj::gptr<double>)q_out;
+    for (int f = 0; f < 4; ++f) {
+        const double v[10] = {q[f].v, q[f].gx, q[f].gy, q[f].gz, q[f].hxx, q[f].hxy, q[f].hxz, q[f].hyy, q[f].hyz, q[f].hzz};
+        for (int o = 0; o < 10; ++o) g[(10 * f + o) * stride] = v[o];
+    }
+}
+
+__device__ __forceinline__ void load_interps(const double* q_in, int64_t stride, srt::Interp<double> q[4]) {
+    const srt::adj::gptr<const double> g = (srt::adj::gptr<const double>)q_in;
+    for (int f = 0; f < 4; ++f) {
+        double v[10];
+        for (int o = 0; o < 10; ++o) v[o] = g[(10 * f + o) * stride];
+        q[f] = srt::Interp<double>{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9]};
+    }
+}
+
+// Right-hand side k at stage state s (both component-major, `stride`), storing
+// the four fields' interpolants at s in q_out for that stage's VJP.  k may be
+// null (the last stage needs only the interpolants).  Not inlined, like
+// stage_vjp.
+__device__ __noinline__ void stage_rhs(const srt::Grids& G_in, const double* __restrict__ s_in, int64_t stride,
+                                       double* __restrict__ q_out, double* __restrict__ k_out) {
+    const srt::Grids G = G_in;
+    double s[19];
+    const srt::adj::gptr<const double> sg = (srt::adj::gptr<const double>)s_in;
+    for (int j = 0; j < 19; ++j) s[j] = sg[j * stride];
+    srt::adj::Stencil st[4];
+    srt::adj::locate_stencils(G, s, st);
+    srt::Interp<double> q[4];
+    srt::adj::interp_fields(G, st, q);
+    store_interps(q_out, stride, q);
+    if (k_out) {
+        srt::adj::RhsTape T;
+        double dy[19];
+        srt::adj::rhs_forward_from(s, q, T, dy);
+        const srt::adj::gptr<double> kg = (srt::adj::gptr<double>)k_out;
+        for (int j = 0; j < 19; ++j) kg[j * stride] = dy[j];
+    }
+}
+
 // dL/ds of one stage given dL/dk (adj_vec[kKbar]), into adj_vec[kSbar],
 // scattering the grid gradient.  The stage state is component j at
-// s[j * stride].  Not inlined: a call boundary keeps the register allocation
-// of the reverse sweep local.
-__device__ __noinline__ void stage_vjp(const AdjArgs& A, const double* __restrict__ s_in, int64_t stride) {
+// s[j * stride], its interpolants (stage_rhs) at q_in.  Not inlined: a call
+// boundary keeps the register allocation of the reverse sweep local.
+__device__ __noinline__ void stage_vjp(const AdjArgs& A, const double* __restrict__ s_in, const double* __restrict__ q_in,
+                                       int64_t stride) {
     const int l = threadIdx.x;
     double s[19], kbar[19], sbar[19];
     const srt::adj::gptr<const double> sg = (srt::adj::gptr<const double>)s_in;
@@ -528,9 +576,11 @@ __device__ __noinline__ void stage_vjp(const AdjArgs& A, const double* __restric
     double* const gd[4] = {A.grad[0], A.grad[1], A.grad[2], A.grad[3]};
     srt::adj::Stencil st[4];
     srt::adj::locate_stencils(G, s, st);
+    srt::Interp<double> q[4];
+    load_interps(q_in, stride, q);
     srt::adj::RhsTape T;
     double dy[19];
-    srt::adj::rhs_forward(G, s, st, T, dy);
+    srt::adj::rhs_forward_from(s, q, T, dy);
     srt::adj::rhs_vjp(G, st, T, kbar, gd, sbar);
     for (int j = 0; j < 19; ++j) adj_vec[kSbar][j][l] = sbar[j];
 }
@@ -610,9 +660,11 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
         for (int i = 0; i < 3; ++i) vec[kLam][i][l] = (w[i] - u[i] * uw) / dn;
     }
 
-    // K step states, then 4 stage states, component-major like the checkpoints
+    // K step states, then 4 stage states, then one stage right-hand side, then
+    // the 4 stages' interpolants, component-major like the checkpoints
     double* const wk = A.work + lane;
     auto W = [&](int q, int j) -> double& { return wk[((int64_t)q * 19 + j) * C]; };
+    auto IQ = [&](int stage) -> double* { return wk + ((int64_t)(K + 5) * 19 + stage * kInterpDoubles) * C; };
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         for (int j = 0; j < 19; ++j) y[j] = ck[((int64_t)sgi * 19 + j) * C];
         const int m = min(K, A.n_steps - sgi * K);
@@ -621,47 +673,38 @@ __global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
             rk4_step(A.G, y);
         }
         for (int st = m - 1; st >= 0; --st) {
-            // stage states of this step: s1 = y, s2 = y + k1/2, s3 = y + k2/2, s4 = y + k3
+            // stage states of this step: s1 = y, s2 = y + k1/2, s3 = y + k2/2,
+            // s4 = y + k3, with each stage's interpolants kept for its VJP
             {
-                double yy[19], k[19], t[19];
-                for (int j = 0; j < 19; ++j) {
-                    yy[j] = W(st, j);
-                    W(K, j) = yy[j];
+                for (int j = 0; j < 19; ++j) W(K, j) = W(st, j);
+                const double c[3] = {0.5, 0.5, 1.0};
+                for (int i = 0; i < 3; ++i) {
+                    stage_rhs(A.G, &W(K + i, 0), C, IQ(i), &W(K + 4, 0));
+                    for (int j = 0; j < 19; ++j) W(K + 1 + i, j) = W(st, j) + c[i] * W(K + 4, j);
                 }
-                rhs19(A.G, yy, k);
-                for (int j = 0; j < 19; ++j) {
-                    t[j] = yy[j] + 0.5 * k[j];
-                    W(K + 1, j) = t[j];
-                }
-                rhs19(A.G, t, k);
-                for (int j = 0; j < 19; ++j) {
-                    t[j] = yy[j] + 0.5 * k[j];
-                    W(K + 2, j) = t[j];
-                }
-                rhs19(A.G, t, k);
-                for (int j = 0; j < 19; ++j) W(K + 3, j) = yy[j] + k[j];
+                stage_rhs(A.G, &W(K + 3, 0), C, IQ(3), nullptr);
             }
             // reverse sweep: y' = y + (k1 + 2 k2 + 2 k3 + k4) / 6
             for (int j = 0; j < 19; ++j) {
                 vec[kYbar][j][l] = vec[kLam][j][l];
                 vec[kKbar][j][l] = vec[kLam][j][l] * (1.0 / 6.0);
             }
-            stage_vjp(A, &W(K + 3, 0), C);  // s4 = y + k3
+            stage_vjp(A, &W(K + 3, 0), IQ(3), C);  // s4 = y + k3
             for (int j = 0; j < 19; ++j) {
                 vec[kYbar][j][l] += vec[kSbar][j][l];
                 vec[kKbar][j][l] = vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 3.0);
             }
-            stage_vjp(A, &W(K + 2, 0), C);  // s3 = y + k2 / 2
+            stage_vjp(A, &W(K + 2, 0), IQ(2), C);  // s3 = y + k2 / 2
             for (int j = 0; j < 19; ++j) {
                 vec[kYbar][j][l] += vec[kSbar][j][l];
                 vec[kKbar][j][l] = 0.5 * vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 3.0);
             }
-            stage_vjp(A, &W(K + 1, 0), C);  // s2 = y + k1 / 2
+            stage_vjp(A, &W(K + 1, 0), IQ(1), C);  // s2 = y + k1 / 2
             for (int j = 0; j < 19; ++j) {
                 vec[kYbar][j][l] += vec[kSbar][j][l];
                 vec[kKbar][j][l] = 0.5 * vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 6.0);
             }
-            stage_vjp(A, &W(K, 0), C);  // s1 = y
+            stage_vjp(A, &W(K, 0), IQ(0), C);  // s1 = y
             for (int j = 0; j < 19; ++j) vec[kLam][j][l] = vec[kYbar][j][l] + vec[kSbar][j][l];
         }
     }
@@ -1116,7 +1159,8 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     const int nseg = (n_steps + K - 1) / K;
     const size_t P = (size_t)ctx->pad, pvox = P * P * P;
     const size_t c = (size_t)count;
-    const size_t n_ck = c * (size_t)(nseg + 1) * 19, n_wk = c * (size_t)(K + 4) * 19, n_gr = map_grad ? 4 * pvox : 0;
+    const size_t n_ck = c * (size_t)(nseg + 1) * 19, n_wk = c * ((size_t)(K + 5) * 19 + 4 * kInterpDoubles),
+                 n_gr = map_grad ? 4 * pvox : 0;
     const size_t need = (n_ck + n_wk + n_gr) * sizeof(double);
     if (need > ctx->adj_bytes) {
         SRT_HIP(hipStreamSynchronize((hipStream_t)stream));
