@@ -1061,6 +1061,9 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown ray order");
     if (precision != SRT_PREC_FP64 && precision != SRT_PREC_MIXED && precision != SRT_PREC_REF)
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
+    // one workgroup per 64 lanes at most: the grid must stay within 2^31 - 1 workgroups
+    if ((count + 63) / 64 > (int64_t)0x7fffffff)
+        return fail(SRT_ERR_UNSUPPORTED, "srt_trace: more rays than one launch holds (2^31 - 1 wavefronts)");
     SRT_HIP(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     // NESTED over the full sky at power-of-two Nside: lanes are NESTED pixels;
@@ -1154,6 +1157,8 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     if (map_grad && (!grad_phi || !grad_shift_xyz))
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: gradient outputs are null");
     if (!map_grad && !maps64) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: nothing to compute");
+    if ((count + kAdjBlock - 1) / kAdjBlock > (int64_t)0x7fffffff)
+        return fail(SRT_ERR_UNSUPPORTED, "srt_trace_adjoint: more rays than one launch holds");
     SRT_HIP(hipSetDevice(ctx->device));
     const int K = std::min(checkpoint_every, n_steps);
     const int nseg = (n_steps + K - 1) / K;

@@ -15,6 +15,8 @@ Bars:
     float interpolation in a different order; the reference itself is built
     with fast math).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -198,6 +200,13 @@ def test_empty_trace_and_errors(srt, tracer):
         tracer.trace_device(NSIDE, 4, o, precision=7)
     assert ei.value.status == 1
     assert float(o.abs().max()) == 0.0  # rejected before any work
+    # a full sky at the largest Nside is more wavefronts than one launch holds:
+    # refused (SRT_ERR_UNSUPPORTED) before the output is touched
+    nmax = 1 << 20
+    rc = srt.lib().srt_trace(tracer._ctx, nmax, 0, 12 * nmax * nmax, None, 1, None, srt.PREC_MIXED,
+                             srt.ORDER_LINEAR, ctypes.c_void_p(o.data_ptr()), None, None)
+    assert rc == 6
+    assert float(o.abs().max()) == 0.0
 
 
 @pytest.mark.parametrize("n", [16, 20])
