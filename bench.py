@@ -298,6 +298,12 @@ def workload_batch(args, pkg, fields, dev):
         print(f"[bench] batch plan {name}: {variants[name]:.1f} ms", file=sys.stderr, flush=True)
     plan.set_order(pkg.ORDER_NESTED, 8)
     tile, blocks = plan.autotune(stream=stream)
+    brick = None
+    if layout == "bicubic":
+        t0 = time.time()
+        brick = plan.autotune_brick(stream=stream)
+        print(f"[bench] batch plan brick autotune: edge {brick} ({time.time() - t0:.1f} s)", file=sys.stderr,
+              flush=True)
     res = {}
     for graph in (False, True):
         plan.launch(stream=stream, graph=graph)
@@ -309,13 +315,13 @@ def workload_batch(args, pkg, fields, dev):
         wall = (time.perf_counter() - t0) / args.steps
         res["graph" if graph else "direct"] = {"ms": wall * 1000, "ray_steps_per_s": total_rs / wall,
                                                "kernel_ms": plan.last_ms()}
-        print(f"[bench] batch plan autotuned ({tile}, {blocks}) graph={graph}: {wall * 1000:.1f} ms",
+        print(f"[bench] batch plan autotuned ({tile}, {blocks}, brick {brick}) graph={graph}: {wall * 1000:.1f} ms",
               file=sys.stderr, flush=True)
     print(json.dumps({"workload": "batch", "instances": 48, "ray_steps": total_rs, "layout": layout,
                       "sequential_launches": seq,
                       "plan_default_config": {"tile_rays": default_cfg[0], "blocks": default_cfg[1]},
                       "plan_variants_ms": variants,
-                      "autotuned": {"tile_rays": tile, "blocks": blocks}, "plan": res}))
+                      "autotuned": {"tile_rays": tile, "blocks": blocks, "brick": brick}, "plan": res}))
     plan.close()
 
 

@@ -204,7 +204,10 @@ typedef struct srt_plan srt_plan;
 /* Builds a plan for tracing all items in one persistent launch: tiles of
  * consecutive rays of one item are handed to workgroups by atomic counters,
  * so items of different grid size, step count and ray count share the GPU.
- * All contexts must be on one device and stay alive while the plan is used. */
+ * All contexts must be on one device and stay alive while the plan is used.
+ * A plan follows its contexts: after srt_upload or srt_set_layout on one of
+ * them, the next launch rebuilds the plan's device table (and its graph) from
+ * the contexts' current buffers, so it traces the new snapshot or layout. */
 int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt_plan** plan);
 int srt_plan_destroy(srt_plan* plan);
 
@@ -226,6 +229,15 @@ int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* to
  * fastest; returns the tile size and block count (srt_plan_get_config and
  * the plan keep the range count). */
 int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks);
+
+/* Times the plan with its contexts' bicubic records rebuilt in column bricks
+ * of edge 1, 4, 8, 16 and 32 (those not larger than the smallest grid edge;
+ * an edge whose records do not fit is skipped) and keeps the fastest: every
+ * context of the plan is left in SRT_LAYOUT_BICUBIC with that edge, so later
+ * srt_trace calls on those contexts use it too.  Needs an SRT_PREC_MIXED plan
+ * whose contexts all hold records.  Outputs do not depend on the edge.
+ * Synchronous (the records are rebuilt between timed launches). */
+int srt_plan_autotune_brick(srt_plan* plan, void* stream, int* brick);
 
 /* Launches the plan on `stream`.  use_graph = 1 captures the launch sequence
  * into a HIP graph on first use (and after srt_plan_configure) and replays it;

@@ -41,7 +41,7 @@ MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "defl
 EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_set_layout",
            "srt_get_layout", "srt_trace",
            "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_set_order", "srt_plan_configure",
-           "srt_plan_get_config", "srt_plan_autotune", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
+           "srt_plan_get_config", "srt_plan_autotune", "srt_plan_autotune_brick", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
            "srt_grid_info", "srt_version")
 
 
@@ -89,6 +89,7 @@ def lib():
         L.srt_plan_set_order.argtypes = [P, i32, i32]
         L.srt_plan_get_config.argtypes = [P, P, P, P]
         L.srt_plan_autotune.argtypes = [P, P, P, P]
+        L.srt_plan_autotune_brick.argtypes = [P, P, P]
         L.srt_plan_launch.argtypes = [P, i32, P]
         L.srt_plan_last_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.srt_last_trace_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
@@ -312,6 +313,13 @@ class Plan:
         st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
         _check(lib().srt_plan_autotune(self._plan, st, ctypes.byref(t), ctypes.byref(b)), "srt_plan_autotune")
         return t.value, b.value
+
+    def autotune_brick(self, stream=None) -> int:
+        """Rebuilds the contexts' bicubic records at each candidate brick edge, keeps the fastest, returns it."""
+        b = ctypes.c_int()
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _check(lib().srt_plan_autotune_brick(self._plan, st, ctypes.byref(b)), "srt_plan_autotune_brick")
+        return b.value
 
     def launch(self, stream=None, graph: bool = False) -> None:
         st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)

@@ -785,6 +785,10 @@ struct srt_context {
     // from SRT_MAX_ALLOC_BYTES at creation, so that callers and tests can
     // exercise the out-of-memory paths on any device
     size_t max_alloc = 0;
+    // bumped whenever a device buffer that a batch plan's table may point at
+    // (padded grids, FP64 copies, records) is freed or replaced, so that plans
+    // rebuild their tables before the next launch
+    uint64_t generation = 0;
 };
 
 namespace {
@@ -813,6 +817,7 @@ int ensure_grids64(srt_context* ctx, hipStream_t st) {
     if (ctx->grids64) return SRT_OK;
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
     SRT_HIP(ctx_malloc(ctx, &ctx->grids64, 4 * pvox * sizeof(double)));
+    ++ctx->generation;
     const int blocks = (int)std::min<int64_t>(((int64_t)(4 * pvox) + 255) / 256, 65536);
     hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, st, ctx->grids, ctx->grids64, (int64_t)(4 * pvox));
     SRT_HIP(hipGetLastError());
@@ -830,24 +835,33 @@ int brick_columns(int n, int shift) {
 }
 
 void free_records(srt_context* ctx) {
-    if (ctx->rec) (void)hipFree(ctx->rec);
+    if (ctx->rec) {
+        (void)hipFree(ctx->rec);
+        ++ctx->generation;
+    }
     ctx->rec = nullptr;
     ctx->rec_field_floats = 0;
 }
 
-// Builds the bicubic cell records of the four fields from the padded cubes.
+// Builds the bicubic cell records of the four fields from the padded cubes,
+// into the current record buffer when it has the right size (a new brick edge
+// that divides the grid edge keeps the size; freeing and allocating hundreds
+// of GB costs seconds).
 int build_records(srt_context* ctx) {
-    free_records(ctx);
     const int n = ctx->n, P = ctx->pad, s = ctx->brick_shift, nb = brick_columns(n, s);
     const size_t cols = (size_t)nb * nb << (2 * s);
     const size_t field = cols * (size_t)P * 16;
-    const hipError_t ea = ctx_malloc(ctx, &ctx->rec, 4 * field * sizeof(float));
-    if (ea != hipSuccess) {
-        return fail(ea == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
-                    "bicubic records: hipMalloc of " + std::to_string(4 * field * sizeof(float)) + " bytes: " +
-                        hipGetErrorName(ea));
+    if (ctx->rec && ctx->rec_field_floats != field) free_records(ctx);
+    if (!ctx->rec) {
+        const hipError_t ea = ctx_malloc(ctx, &ctx->rec, 4 * field * sizeof(float));
+        if (ea != hipSuccess) {
+            return fail(ea == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
+                        "bicubic records: hipMalloc of " + std::to_string(4 * field * sizeof(float)) + " bytes: " +
+                            hipGetErrorName(ea));
+        }
     }
     ctx->rec_field_floats = field;
+    ++ctx->generation;
     const size_t pvox = (size_t)P * P * P;
     const int64_t total = (int64_t)n * n * P;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
@@ -955,6 +969,7 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
     SRT_HIP(hipSetDevice(ctx->device));
     const int n = (int)nx, P = n + 3;
     const size_t nvox = (size_t)n * n * n, pvox = (size_t)P * P * P;
+    ++ctx->generation;
     if (ctx->grids) {
         SRT_HIP(hipFree(ctx->grids));
         ctx->grids = nullptr;
@@ -1262,6 +1277,7 @@ struct srt_plan {
     void* order_buf = nullptr;                 // per-item lane -> slot maps + sort space
     size_t order_bytes = 0;
     std::vector<const uint32_t*> item_map;     // per item: its map in order_buf, or null
+    std::vector<uint64_t> item_gen;            // per item: its context's generation when the table was built
     int tile_rays = 64;  // one 64-ray group per claim: in-flight rays of an XCD stay as compact as a trace launch's
     int blocks = 0;
     int64_t total_tiles = 0;
@@ -1316,8 +1332,12 @@ int plan_build_orders(srt_plan* p) {
     return SRT_OK;
 }
 
-// (re)builds the device instance table for the plan's tile size
+// (re)builds the device instance table for the plan's tile size from the
+// contexts' current buffers
 int plan_build_table(srt_plan* p) {
+    for (int i = 0; i < p->n_inst; ++i)
+        if (!p->items[i].ctx->grids)
+            return fail(SRT_ERR_NOT_UPLOADED, "plan item " + std::to_string(i) + ": context without snapshot");
     std::vector<BatchInst> h(p->n_inst);
     int64_t tiles = 0;
     p->bicubic = p->precision == SRT_PREC_MIXED;
@@ -1355,7 +1375,20 @@ int plan_build_table(srt_plan* p) {
     SRT_HIP(hipSetDevice(p->device));
     SRT_HIP(hipDeviceSynchronize());
     SRT_HIP(hipMemcpy(p->d_inst, h.data(), h.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+    p->item_gen.resize(p->n_inst);
+    for (int i = 0; i < p->n_inst; ++i) p->item_gen[i] = p->items[i].ctx->generation;
     return SRT_OK;
+}
+
+// Rebuilds the table (and drops the captured graph, whose kernel may change
+// with the layout or grid size) if any context's buffers were replaced since
+// it was built: a new snapshot, records rebuilt or dropped, FP64 copies made.
+int plan_refresh(srt_plan* p) {
+    bool stale = p->item_gen.size() != (size_t)p->n_inst;
+    for (int i = 0; i < p->n_inst && !stale; ++i) stale = p->items[i].ctx->generation != p->item_gen[i];
+    if (!stale) return SRT_OK;
+    plan_drop_graph(p);
+    return plan_build_table(p);
 }
 
 // batch_kernel<F> for plan p, specialised when every item's grid edge is a power of two
@@ -1503,6 +1536,8 @@ int srt_plan_launch(srt_plan* plan, int use_graph, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     if (use_graph && !st)
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_launch: graph replay needs a non-default stream");
+    const int rc_refresh = plan_refresh(plan);
+    if (rc_refresh != SRT_OK) return rc_refresh;
     if (use_graph && plan->exec && plan->graph_stream != st) plan_drop_graph(plan);
     if (use_graph && !plan->exec) {
         SRT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1573,6 +1608,54 @@ int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks)
     if (rc != SRT_OK) return rc;
     if (tile_rays) *tile_rays = bt;
     if (blocks) *blocks = bb;
+    return SRT_OK;
+}
+
+int srt_plan_autotune_brick(srt_plan* plan, void* stream, int* brick) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_autotune_brick: null plan");
+    SRT_HIP(hipSetDevice(plan->device));
+    int rc = plan_refresh(plan);
+    if (rc != SRT_OK) return rc;
+    if (!plan->bicubic)
+        return fail(SRT_ERR_INVALID_ARGUMENT,
+                    "srt_plan_autotune_brick: needs a mixed-precision plan whose contexts hold bicubic records");
+    std::vector<srt_context*> ctxs;
+    int min_n = 1 << 30;
+    for (const auto& it : plan->items) {
+        if (std::find(ctxs.begin(), ctxs.end(), it.ctx) == ctxs.end()) ctxs.push_back(it.ctx);
+        min_n = std::min(min_n, it.ctx->n);
+    }
+    // every context in bricks of edge b; a context whose records do not fit
+    // falls back to the padded grids, and the candidate is skipped
+    auto set_all = [&](int b) {
+        for (srt_context* c : ctxs) {
+            const int r = srt_set_layout(c, SRT_LAYOUT_BICUBIC, b);
+            if (r != SRT_OK) return r;
+        }
+        return (int)SRT_OK;
+    };
+    const int candidates[] = {1, 4, 8, 16, 32};
+    float best = 0.0f;
+    int bb = 0;
+    for (int b : candidates) {
+        if (b > min_n) continue;
+        rc = set_all(b);
+        if (rc == SRT_ERR_OUT_OF_MEMORY) continue;
+        if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);  // warm-up; rebuilds the table
+        float ms = 0.0f;
+        if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);
+        if (rc == SRT_OK) rc = srt_plan_last_ms(plan, &ms);
+        if (rc != SRT_OK) return rc;
+        if (!bb || ms < best) {
+            best = ms;
+            bb = b;
+        }
+    }
+    if (!bb) return fail(SRT_ERR_OUT_OF_MEMORY, "srt_plan_autotune_brick: no brick edge fits");
+    rc = set_all(bb);
+    if (rc == SRT_OK) rc = plan_refresh(plan);
+    if (rc != SRT_OK) return rc;
+    if (brick) *brick = bb;
     return SRT_OK;
 }
 

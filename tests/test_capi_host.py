@@ -50,6 +50,7 @@ def test_null_arguments_rejected_without_device_calls(srt):
     assert L.srt_get_layout(None, None, None, None) == 1
     assert L.srt_plan_set_order(None, 1, 8) == 1
     assert L.srt_plan_autotune(None, None, None, None) == 1
+    assert L.srt_plan_autotune_brick(None, None, None) == 1
     assert b"null" in L.srt_last_error()
 
 

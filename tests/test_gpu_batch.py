@@ -177,3 +177,97 @@ def test_plan_many_tiles_per_workgroup(srt, fields, tile, blocks):
         for it, r in zip(items[1:], ref_b):
             assert np.array_equal(it["out"].cpu().numpy(), r.cpu().numpy())
     plan.close()
+
+
+def _tracer(srt, fields, n, seed, layout=None, brick=16):
+    import torch
+    phi, shift = fields.snapshot(n, seed)
+    tr = srt.Tracer(0)
+    if layout is not None:
+        tr.set_layout(layout, brick)
+    tr.upload(phi.to("cuda:0"), fields.interleave_shift(shift).to("cuda:0"))
+    return tr
+
+
+def _items(tracers):
+    import torch
+    a, b = tracers
+    return [dict(tracer=a, nside=4, pix_begin=0, count=192, n_steps=6, observer=None,
+                 out=torch.zeros(6 * 192, dtype=torch.float32, device="cuda:0")),
+            dict(tracer=b, nside=8, pix_begin=3 * 64, count=64, n_steps=9, observer=(9.5, 20.25, 11.0),
+                 out=torch.zeros(6 * 64, dtype=torch.float32, device="cuda:0")),
+            dict(tracer=a, nside=8, pix_begin=700, count=68, n_steps=5, observer=(3.0, 4.0, 5.0),
+                 out=torch.zeros(6 * 68, dtype=torch.float32, device="cuda:0"))]
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_plan_follows_reupload_and_layout_changes(srt, fields, prec):
+    """A plan built before a context gets a new snapshot (of another size) or
+    another layout traces the context's current buffers at its next launch,
+    direct or from a graph: the same outputs as fresh single traces."""
+    import torch
+    print(f"[gpu] batch plan after re-upload / layout change, {prec}", flush=True)
+    p = srt.PRECISIONS[prec]
+    a = _tracer(srt, fields, 24, 301, srt.LAYOUT_BICUBIC)
+    b = _tracer(srt, fields, 32, 302, srt.LAYOUT_BICUBIC)
+    items = _items((a, b))
+    plan = srt.Plan(items, precision=p)
+    s = torch.cuda.Stream()
+
+    def run_and_check(graph, only=(0, 1, 2)):
+        ref = expected(srt, items, p)
+        for rep in range(2):
+            for it in items:
+                it["out"].fill_(float("nan"))
+            torch.cuda.synchronize()
+            with torch.cuda.stream(s):
+                plan.launch(stream=s, graph=graph)
+            s.synchronize()
+            check([items[i] for i in only], [ref[i] for i in only])
+
+    run_and_check(False)
+    run_and_check(True)
+    phi, shift = fields.snapshot(20, 303)  # new snapshot, new grid edge: every buffer of `a` is replaced
+    a.upload(phi.to("cuda:0"), fields.interleave_shift(shift).to("cuda:0"))
+    run_and_check(True)
+    b.set_layout(srt.LAYOUT_BICUBIC, 4)  # records rebuilt in 4 x 4 bricks
+    run_and_check(True)
+    run_and_check(False)
+    # records of `b` dropped: a mixed plan then runs the padded-grid kernel for
+    # every item, so only `b`'s item matches a single trace until `a` drops its
+    # records too
+    b.set_layout(srt.LAYOUT_TAPS)
+    run_and_check(True, only=(1,) if prec == "mixed" else (0, 1, 2))
+    a.set_layout(srt.LAYOUT_TAPS)
+    run_and_check(True)
+    run_and_check(False)
+    plan.close()
+    a.close()
+    b.close()
+
+
+def test_plan_autotune_brick(srt, fields):
+    import torch
+    print("[gpu] batch plan brick autotune", flush=True)
+    a = _tracer(srt, fields, 16, 311, srt.LAYOUT_BICUBIC, 1)
+    b = _tracer(srt, fields, 32, 312, srt.LAYOUT_BICUBIC, 8)
+    items = _items((a, b))
+    plan = srt.Plan(items, precision=srt.PREC_MIXED)
+    brick = plan.autotune_brick()
+    assert brick in (1, 4, 8, 16)  # edges above the smallest grid edge (16) are not tried
+    for tr in (a, b):
+        lay, br, nbytes = tr.layout()
+        assert (lay, br) == (srt.LAYOUT_BICUBIC, brick) and nbytes > 0
+    ref = expected(srt, items, srt.PREC_MIXED)
+    for it in items:
+        it["out"].fill_(float("nan"))
+    plan.launch()
+    torch.cuda.synchronize()
+    check(items, ref)
+    plan.close()
+    fp64 = srt.Plan(items, precision=srt.PREC_FP64)
+    with pytest.raises(srt.SrtError):
+        fp64.autotune_brick()  # only mixed plans read records
+    fp64.close()
+    a.close()
+    b.close()
