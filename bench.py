@@ -10,7 +10,8 @@ Printed: one JSON line with ray-steps/s of the mixed-precision kernel (the
 headline `value`, reading the bicubic cell records), the plain FP64 kernel's
 rate and the mixed kernel on the raw padded cubes beside it, the roofline of
 the dominant kernel, the per-map error of every kernel against the host FP64
-oracle on a seeded 1% pixel subset, and the oracle's own CPU rate.
+oracle on a seeded 1% pixel subset, the full-sky dA and shear errors of the
+other kernels against the FP64 kernel, and the oracle's own CPU rate.
 """
 from __future__ import annotations
 
@@ -73,6 +74,15 @@ def parse():
                    help="first run one device copy of 1 GiB (a known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     return p.parse_args()
+
+
+def full_sky_errors(test, ref):
+    """dA and shear errors of a full-sky trace (maps 0-2, on the device) against the FP64 kernel's."""
+    t, r = test.double(), ref.double()
+    rel = (t[0] - r[0]).abs() / r[0].abs()
+    de2 = (t[1] - r[1]) ** 2 + (t[2] - r[2]) ** 2
+    return {"dA_max_rel": rel.max().item(), "dA_pixels_above_1e-5": int((rel > 1e-5).sum().item()),
+            "shear_rms_rel": (de2.mean() / (r[1] ** 2 + r[2] ** 2).mean()).sqrt().item()}
 
 
 def map_errors(test, ref):
@@ -409,6 +419,8 @@ def main():
     # kernels on the padded cubes first, so the bicubic records are built once
     names.sort(key=lambda k: KERNELS[k][1] == "bicubic")
     layout_s = {}
+    fp64_maps = None
+    full_sky = {}
     for name in names:
         prec_name, layout = KERNELS[name]
         prec = pkg.PRECISIONS[prec_name]
@@ -436,6 +448,10 @@ def main():
             "kernel_ms_min": float(np.min(kernel_ms)),
         }
         maps_sub[name] = out.view(6, rays)[:, subset_t].cpu().numpy()
+        if name == "fp64":
+            fp64_maps = out.view(6, rays)[:3].clone()
+        elif fp64_maps is not None:
+            full_sky[name] = full_sky_errors(out.view(6, rays)[:3], fp64_maps)
         print(f"[bench] {name}: {ms_per_step:.1f} ms/step, kernel {avg_kernel_ms:.1f} ms, "
               f"{results[name]['ray_steps_per_s']:.3e} ray-steps/s", file=sys.stderr, flush=True)
 
@@ -522,6 +538,7 @@ def main():
                    "mixed_taps": "padded periodic cubes", "fp64": "padded periodic cubes (FP64 copies)",
                    "ref": "padded periodic cubes", "build_s": layout_s},
         "map_error_vs_fp64_oracle": accuracy,
+        "full_sky_error_vs_fp64_kernel": full_sky,
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "upload_s": upload_s,

@@ -1,0 +1,175 @@
+"""GPU parity at the drop-in scale: a periodic 512^3 snapshot (potential rms
+1e-5, shift rms 1e-7, P(k) ~ k^-3, seed 2), observer at the grid centre,
+Nside 1024 (12,582,912 rays), 256 steps; mixed traces read bicubic records in
+16 x 16 column bricks.
+
+The oracle is too slow for the whole sky, so parity with it is checked on a
+seeded subset of 1536 pixels, and the full sky through properties that do not
+depend on size:
+  * FP64 kernel vs FP64 oracle on the subset: final states within 1e-10 of
+    each component's scale, maps within 2 float ulps; the full-sky FP64 launch
+    (NESTED lanes) equals the subset's pixel-list launch bit for bit;
+  * mixed kernel vs FP64 oracle on the subset: dA max per-pixel relative
+    error <= 1e-5, shear rms(|d eps|)/rms(|eps|) <= 1e-4, other maps
+    rms(d)/rms <= 1e-3;
+  * mixed kernel vs FP64 kernel over the full sky: the same rms bounds; dA
+    relative error above 1e-5 in at most 1e-5 of the pixels and nowhere above
+    1e-3; and every pixel above 1e-5 is one whose FP64 value is itself
+    unstable: the FP64 oracle, with the observer moved by at most 1e-8 cells,
+    moves by at least half the mixed kernel's deviation there.  (Such pixels
+    exist because the Catmull-Rom Hessian jumps across cell faces: a ray that
+    passes within ~1e-9 cells of a face takes one side or the other.  On this
+    snapshot 13 of 12.6 million pixels exceed 1e-5, up to 1.4e-4, each moving
+    the oracle by exactly that amount under a shift of <= 3e-9 cells; random
+    pixels move by <= 1 float ulp under 1e-8.);
+  * the 12 RING batches, as separate launches and as one persistent plan,
+    and LINEAR vs NESTED lane order: bit-identical to the full-sky launch.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N, NSIDE, STEPS, SEED = 512, 1024, 256, 2
+RAYS = 12 * NSIDE * NSIDE
+
+
+def map_errors(test, ref):
+    out = {"dA_max_rel": float(np.max(np.abs(test[0].astype(np.float64) - ref[0]) / np.abs(ref[0])))}
+    de = np.hypot(test[1].astype(np.float64) - ref[1], test[2].astype(np.float64) - ref[2])
+    out["shear_rms"] = float(np.sqrt(np.mean(de ** 2)) /
+                             np.sqrt(np.mean(ref[1].astype(np.float64) ** 2 + ref[2].astype(np.float64) ** 2)))
+    for m, name in ((3, "rotation"), (4, "deflection1"), (5, "deflection2")):
+        rms = np.sqrt(np.mean(ref[m].astype(np.float64) ** 2))
+        d = test[m].astype(np.float64) - ref[m]
+        out[name + "_rms_rel"] = float(np.sqrt(np.mean(d ** 2)) / rms)
+    return out
+
+
+def check_bounds(e):
+    assert e["dA_max_rel"] <= 1e-5
+    assert e["shear_rms"] <= 1e-4
+    for name in ("rotation", "deflection1", "deflection2"):
+        assert e[name + "_rms_rel"] <= 1e-3, name
+
+
+@pytest.fixture(scope="module")
+def big(srt, fields):
+    import torch
+    print(f"[gpu] full size: {N}^3 snapshot on the device, bicubic records", flush=True)
+    phi, shift = fields.snapshot(N, SEED, device="cuda:0")
+    tr = srt.Tracer(0)
+    tr.set_layout(srt.LAYOUT_BICUBIC, 16)
+    tr.upload(phi, fields.interleave_shift(shift))
+    host = (phi.cpu().numpy(), shift.cpu().numpy())
+    del phi, shift
+    torch.cuda.empty_cache()
+    yield tr, host
+    tr.close()
+
+
+@pytest.fixture(scope="module")
+def ofields(oracle, big):
+    _, (phi, shift) = big
+    return oracle.Fields(phi, shift)
+
+
+@pytest.fixture(scope="module")
+def subset_ref(oracle, ofields):
+    pix = np.sort(np.random.default_rng(8).choice(RAYS, 1536, replace=False))
+    print(f"[gpu] full size: FP64 oracle on {pix.size} pixels x {STEPS} steps", flush=True)
+    maps, _, state = oracle.trace(ofields, NSIDE, pix, STEPS, form=oracle.FORM_FP64, return_state=True)
+    return pix, maps, state
+
+
+@pytest.fixture(scope="module")
+def full_sky(srt, big):
+    """Full-sky FP64 and mixed maps (6, RAYS) on the device, NESTED lanes."""
+    import torch
+    tr, _ = big
+    out = {}
+    for name, prec in (("fp64", srt.PREC_FP64), ("mixed", srt.PREC_MIXED)):
+        print(f"[gpu] full size: {name} full sky", flush=True)
+        o = torch.empty(6 * RAYS, dtype=torch.float32, device="cuda:0")
+        tr.trace_device(NSIDE, STEPS, o, precision=prec, order=srt.ORDER_NESTED)
+        torch.cuda.synchronize()
+        out[name] = o.view(6, RAYS)
+    return out
+
+
+@pytest.fixture(scope="module")
+def subset_fp64(srt, big, subset_ref):
+    tr, _ = big
+    return tr.trace(NSIDE, STEPS, pix=subset_ref[0], precision=srt.PREC_FP64, return_state=True)
+
+
+def test_full_size_fp64_matches_oracle_on_subset(subset_ref, subset_fp64):
+    _, ref, ref_state = subset_ref
+    maps, state = subset_fp64
+    scale = np.abs(ref_state).max(axis=0)
+    scale[scale == 0] = 1.0
+    err = float((np.abs(state - ref_state) / scale).max())
+    print(f"full size fp64 state error vs oracle {err:.3e}")
+    assert err < 1e-10
+    np.testing.assert_allclose(maps, ref, rtol=2.5e-7, atol=1e-30)
+
+
+def test_full_size_mixed_error_bounds(oracle, ofields, subset_ref, subset_fp64, full_sky):
+    pix, ref, _ = subset_ref
+    mixed = full_sky["mixed"].cpu().numpy()
+    e_oracle = map_errors(mixed[:, pix], ref)
+    print("full size mixed vs fp64 oracle (subset):", e_oracle)
+    check_bounds(e_oracle)
+    fp64 = full_sky["fp64"].cpu().numpy()
+    assert np.array_equal(fp64[:, pix], subset_fp64[0])  # NESTED full sky vs LINEAR pixel list
+    e_gpu = map_errors(mixed, fp64)
+    print("full size mixed vs fp64 kernel (full sky):", e_gpu)
+    assert e_gpu["shear_rms"] <= 1e-4
+    for name in ("rotation", "deflection1", "deflection2"):
+        assert e_gpu[name + "_rms_rel"] <= 1e-3, name
+    rel = np.abs(mixed[0].astype(np.float64) - fp64[0]) / np.abs(fp64[0].astype(np.float64))
+    out = np.nonzero(rel > 1e-5)[0]
+    print(f"full size: {out.size} pixels with dA error > 1e-5, max {rel.max():.3e}")
+    assert out.size <= 1e-5 * RAYS and rel.max() <= 1e-3
+    if out.size == 0:
+        return
+    o0, _ = oracle.trace(ofields, NSIDE, out, STEPS, form=oracle.FORM_FP64)
+    np.testing.assert_allclose(o0, fp64[:, out], rtol=2.5e-7, atol=1e-30)
+    moved = np.zeros(out.size)
+    rng = np.random.default_rng(1)
+    for d in (1e-9, 3e-9, 1e-8):
+        for _ in range(8):
+            u = rng.normal(size=3)
+            obs = 0.5 * N + d * u / np.linalg.norm(u)
+            o1, _ = oracle.trace(ofields, NSIDE, out, STEPS, observer=obs, form=oracle.FORM_FP64)
+            moved = np.maximum(moved, np.abs(o1[0].astype(np.float64) - o0[0]) / np.abs(o0[0].astype(np.float64)))
+    print("oracle change under observer shifts / mixed deviation:", np.round(moved / rel[out], 3).tolist())
+    assert np.all(moved >= 0.5 * rel[out])
+
+
+def test_full_size_batches_plan_and_lane_orders(srt, big, full_sky):
+    import torch
+    tr, _ = big
+    mixed = full_sky["mixed"]
+    per = NSIDE * NSIDE
+    items = []
+    for b in range(12):
+        o = torch.full((6 * per,), float("nan"), dtype=torch.float32, device="cuda:0")
+        tr.trace_device(NSIDE, STEPS, o, pix_begin=b * per, count=per, precision=srt.PREC_MIXED,
+                        order=srt.ORDER_NESTED)
+        torch.cuda.synchronize()
+        assert torch.equal(o.view(6, per), mixed[:, b * per:(b + 1) * per]), b
+        items.append(dict(tracer=tr, nside=NSIDE, pix_begin=b * per, count=per, n_steps=STEPS,
+                          out=torch.full((6 * per,), float("nan"), dtype=torch.float32, device="cuda:0")))
+    print("[gpu] full size: 12 batches as one plan", flush=True)
+    plan = srt.Plan(items, precision=srt.PREC_MIXED)
+    plan.launch()
+    torch.cuda.synchronize()
+    for b, it in enumerate(items):
+        assert torch.equal(it["out"].view(6, per), mixed[:, b * per:(b + 1) * per]), b
+    plan.close()
+    print("[gpu] full size: linear lane order", flush=True)
+    lin = torch.empty(6 * RAYS, dtype=torch.float32, device="cuda:0")
+    tr.trace_device(NSIDE, STEPS, lin, precision=srt.PREC_MIXED, order=srt.ORDER_LINEAR)
+    torch.cuda.synchronize()
+    assert torch.equal(lin.view(6, RAYS), mixed)
