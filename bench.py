@@ -68,7 +68,7 @@ def parse():
                    help="headline: 512^3 / Nside 1024 / 256 steps (the JSON line the driver reads); "
                         "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
                         "Nside 64 / 64 steps gradient with finite-difference check; batch: 48 uneven "
-                        "instances in one persistent launch; layouts: the headline workload's mixed "
+                        "instances in one plan launch (persistent and dispatch schedules); layouts: the headline workload's mixed "
                         "kernel on the padded cubes and on bicubic records of several brick edges")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size kernel that scales the "
@@ -109,7 +109,7 @@ def synth(fields, n, seed, dev):
 def workload_large(args, pkg, fields, dev):
     """1024^3 potential + 3 x 1024^3 shift, Nside 2048 full sky, 512 steps, mixed, one launch per
     layout and lane order: bicubic cell records in 16x16 column bricks (also as the 12 RING batches in
-    one persistent plan launch) and in row-major columns (when they fit in device memory), then the
+    one plan launch, persistent and dispatch schedules) and in row-major columns (when they fit in device memory), then the
     padded grids."""
     import torch
     n, nside, n_steps = 1024, 2048, 512
@@ -141,17 +141,26 @@ def workload_large(args, pkg, fields, dev):
     try:
         tr.set_layout(pkg.LAYOUT_BICUBIC, 16)
         run("bicubic_brick16", both)
-        # the 12 RING batches of the sky as 12 items of one persistent launch (default configuration:
-        # 64-ray tiles, 2 blocks per CU, NESTED lanes within each batch, 8 XCD tile ranges)
+        # the 12 RING batches of the sky as 12 items of one persistent launch (64-ray tiles, 2 blocks
+        # per CU, NESTED lanes within each batch, 8 XCD tile ranges)
         per = nside * nside
         items = [dict(tracer=tr, nside=nside, pix_begin=b * per, count=per, n_steps=n_steps,
                       out=out[6 * per * b: 6 * per * (b + 1)]) for b in range(12)]
         plan = pkg.Plan(items, precision=pkg.PREC_MIXED)
+        plan.set_schedule(pkg.SCHEDULE_PERSISTENT)
         plan.launch(stream=stream)
         ms = plan.last_ms()
         res["bicubic_brick16"]["plan_12_batches"] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0),
                                                     "config": plan.config()[:2]}
         print(f"[bench] large bicubic_brick16, 12 batches in one persistent launch: {ms:.0f} ms",
+              file=sys.stderr, flush=True)
+        # the same plan under the dispatch schedule (one 64-lane workgroup per tile)
+        plan.set_schedule(pkg.SCHEDULE_DISPATCH)
+        plan.launch(stream=stream)
+        ms = plan.last_ms()
+        res["bicubic_brick16"]["plan_12_batches_dispatch"] = {"kernel_ms": ms,
+                                                             "ray_steps_per_s": rays * n_steps / (ms / 1000.0)}
+        print(f"[bench] large bicubic_brick16, 12 batches in one dispatch-schedule launch: {ms:.0f} ms",
               file=sys.stderr, flush=True)
         plan.close()
         del items, plan
@@ -241,7 +250,8 @@ def workload_adjoint(args, pkg, fields, dev):
 def workload_batch(args, pkg, fields, dev):
     """48 instances, grids from {128,192,256,384}^3, observers uniform in the box, n_steps in
     [64, N/2], Nside from {128,256,512}, one random RING batch each (rng 7, seeds 1000 + 4 i).
-    Mixed precision; bicubic cell records for every instance when they all fit in device memory."""
+    Mixed precision; bicubic cell records for every instance when they all fit in device memory.
+    Times 48 separate launches, the plan under both schedules, then the autotuned plan."""
     import numpy as np
     import torch
     rng = np.random.default_rng(7)
@@ -297,6 +307,7 @@ def workload_batch(args, pkg, fields, dev):
     plan = pkg.Plan(items, precision=pkg.PREC_MIXED)
     default_cfg = plan.config()[:2]
     variants = {}
+    plan.set_schedule(pkg.SCHEDULE_PERSISTENT)
     for name, order, groups in (("linear_1group", pkg.ORDER_LINEAR, 1), ("nested_1group", pkg.ORDER_NESTED, 1),
                                 ("nested_8groups", pkg.ORDER_NESTED, 8)):
         plan.set_order(order, groups)
@@ -306,8 +317,19 @@ def workload_batch(args, pkg, fields, dev):
         stream.synchronize()
         variants[name] = plan.last_ms()
         print(f"[bench] batch plan {name}: {variants[name]:.1f} ms", file=sys.stderr, flush=True)
+    plan.set_schedule(pkg.SCHEDULE_DISPATCH)
+    for name, groups in (("dispatch_nested_1group", 1), ("dispatch_nested_8groups", 8)):
+        plan.set_order(pkg.ORDER_NESTED, groups)
+        plan.launch(stream=stream)
+        stream.synchronize()
+        plan.launch(stream=stream)
+        stream.synchronize()
+        variants[name] = plan.last_ms()
+        print(f"[bench] batch plan {name}: {variants[name]:.1f} ms", file=sys.stderr, flush=True)
+    plan.set_schedule(pkg.SCHEDULE_PERSISTENT)
     plan.set_order(pkg.ORDER_NESTED, 8)
     tile, blocks = plan.autotune(stream=stream)
+    schedule = "dispatch" if plan.schedule() == pkg.SCHEDULE_DISPATCH else "persistent"
     brick = None
     if layout == "bicubic":
         t0 = time.time()
@@ -325,13 +347,15 @@ def workload_batch(args, pkg, fields, dev):
         wall = (time.perf_counter() - t0) / args.steps
         res["graph" if graph else "direct"] = {"ms": wall * 1000, "ray_steps_per_s": total_rs / wall,
                                                "kernel_ms": plan.last_ms()}
-        print(f"[bench] batch plan autotuned ({tile}, {blocks}, brick {brick}) graph={graph}: {wall * 1000:.1f} ms",
+        print(f"[bench] batch plan autotuned ({schedule}, {tile}, {blocks}, brick {brick}) graph={graph}: "
+              f"{wall * 1000:.1f} ms",
               file=sys.stderr, flush=True)
     print(json.dumps({"workload": "batch", "instances": 48, "ray_steps": total_rs, "layout": layout,
                       "sequential_launches": seq,
                       "plan_default_config": {"tile_rays": default_cfg[0], "blocks": default_cfg[1]},
                       "plan_variants_ms": variants,
-                      "autotuned": {"tile_rays": tile, "blocks": blocks, "brick": brick}, "plan": res}))
+                      "autotuned": {"schedule": schedule, "tile_rays": tile, "blocks": blocks, "brick": brick},
+                      "plan": res}))
     plan.close()
 
 

@@ -182,7 +182,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
                       int n_steps, const double* observer, int checkpoint_every, const double* map_grad,
                       double* maps64, double* grad_phi, double* grad_shift_xyz, void* stream);
 
-/* ---- batched persistent launch (no counterpart in the reference, whose unit
+/* ---- batched launch (no counterpart in the reference, whose unit
  * of work is one process per snapshot and batch, README.md:41-42) ---- */
 
 /* One instance of a batched trace: a context (one uploaded snapshot), a RING
@@ -201,9 +201,9 @@ typedef struct {
 
 typedef struct srt_plan srt_plan;
 
-/* Builds a plan for tracing all items in one persistent launch: tiles of
- * consecutive rays of one item are handed to workgroups by atomic counters,
- * so items of different grid size, step count and ray count share the GPU.
+/* Builds a plan for tracing all items in one launch: tiles of consecutive
+ * rays of one item are handed to workgroups (srt_plan_schedule), so items of
+ * different grid size, step count and ray count share the GPU.
  * All contexts must be on one device and stay alive while the plan is used.
  * A plan follows its contexts: after srt_upload or srt_set_layout on one of
  * them, the next launch rebuilds the plan's device table (and its graph) from
@@ -213,10 +213,29 @@ int srt_plan_destroy(srt_plan* plan);
 
 /* Lane order within each item (srt_ray_order; default SRT_ORDER_NESTED: each
  * item's rays sorted into compact sky patches once, when set) and the number
- * of contiguous tile ranges the launch splits the work into (1..64; default
- * 8: a workgroup starts on the range of its XCD, so neighbouring tiles share
- * an L2, then helps the other ranges).  Outputs do not depend on either. */
+ * of tile ranges the launch splits the work into (1..64; default 8, one per
+ * XCD, so neighbouring tiles share an L2; a persistent workgroup then helps
+ * the other ranges).  Outputs do not depend on either. */
 int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups);
+
+/* How a plan's launch hands out its rays. */
+typedef enum {
+    /* Persistent 256-lane workgroups claim tiles from atomic counters
+     * (srt_plan_configure sets the tile size and workgroup count). */
+    SRT_SCHEDULE_PERSISTENT = 0,
+    /* One workgroup per tile of consecutive lanes of one item (64 lanes for
+     * mixed plans over bicubic records, else 256), balanced by the hardware
+     * dispatcher as a single srt_trace launch is; each tile range (see
+     * srt_plan_set_order) holds the same share of every item. */
+    SRT_SCHEDULE_DISPATCH = 1
+} srt_plan_schedule;
+
+/* Schedule of the plan's launches (srt_plan_schedule; default
+ * SRT_SCHEDULE_DISPATCH, which measured faster on every workload tried: it
+ * has no claim barriers and runs one-wavefront workgroups for mixed plans over
+ * records).  Outputs do not depend on it. */
+int srt_plan_set_schedule(srt_plan* plan, int schedule);
+int srt_plan_get_schedule(srt_plan* plan, int* schedule);
 
 /* Rays per wavefront per claim (multiple of 64; default 64: a 256-lane
  * workgroup claims 4 x 64 neighbouring rays at a time) and number of
@@ -224,10 +243,11 @@ int srt_plan_set_order(srt_plan* plan, int ray_order, int tile_groups);
 int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks);
 int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps);
 
-/* Times every (tile ranges, tile_rays, blocks per CU) candidate (1 or 8
- * ranges, 64..1024 rays, 1/2/4 blocks per CU) on `stream` and keeps the
- * fastest; returns the tile size and block count (srt_plan_get_config and
- * the plan keep the range count). */
+/* Times every persistent (tile ranges, tile_rays, blocks per CU) candidate
+ * (1 or 8 ranges, 64..1024 rays, 1/2/4 blocks per CU) and the dispatch
+ * schedule with 1 or 8 ranges on `stream`, and keeps the fastest; returns the
+ * persistent tile size and block count (srt_plan_get_config), and the plan
+ * keeps the schedule (srt_plan_get_schedule) and range count. */
 int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks);
 
 /* Times the plan with its contexts' bicubic records rebuilt in column bricks

@@ -32,6 +32,9 @@ SHIFT_F16_SCALED = 1
 ORDER_LINEAR = 0
 ORDER_NESTED = 1
 
+SCHEDULE_PERSISTENT = 0
+SCHEDULE_DISPATCH = 1
+
 LAYOUT_TAPS = 0
 LAYOUT_BICUBIC = 1
 
@@ -41,7 +44,7 @@ MAP_NAMES = ("dAmap", "ellipticity1map", "ellipticity2map", "rotationmap", "defl
 EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_upload", "srt_set_layout",
            "srt_get_layout", "srt_trace",
            "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_set_order", "srt_plan_configure",
-           "srt_plan_get_config", "srt_plan_autotune", "srt_plan_autotune_brick", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
+           "srt_plan_get_config", "srt_plan_set_schedule", "srt_plan_get_schedule", "srt_plan_autotune", "srt_plan_autotune_brick", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
            "srt_grid_info", "srt_version")
 
 
@@ -89,6 +92,8 @@ def lib():
         L.srt_plan_set_order.argtypes = [P, i32, i32]
         L.srt_plan_get_config.argtypes = [P, P, P, P]
         L.srt_plan_autotune.argtypes = [P, P, P, P]
+        L.srt_plan_set_schedule.argtypes = [P, i32]
+        L.srt_plan_get_schedule.argtypes = [P, P]
         L.srt_plan_autotune_brick.argtypes = [P, P, P]
         L.srt_plan_launch.argtypes = [P, i32, P]
         L.srt_plan_last_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
@@ -298,6 +303,15 @@ class Plan:
     def set_order(self, order: int = ORDER_NESTED, tile_groups: int = 8) -> None:
         """Lane order within each item and the number of XCD tile ranges (outputs do not change)."""
         _check(lib().srt_plan_set_order(self._plan, order, tile_groups), "srt_plan_set_order")
+
+    def set_schedule(self, schedule: int) -> None:
+        """SCHEDULE_PERSISTENT (work-stealing workgroups) or SCHEDULE_DISPATCH (one workgroup per tile)."""
+        _check(lib().srt_plan_set_schedule(self._plan, schedule), "srt_plan_set_schedule")
+
+    def schedule(self) -> int:
+        v = ctypes.c_int()
+        _check(lib().srt_plan_get_schedule(self._plan, ctypes.byref(v)), "srt_plan_get_schedule")
+        return v.value
 
     def configure(self, tile_rays: int, blocks: int) -> None:
         _check(lib().srt_plan_configure(self._plan, tile_rays, blocks), "srt_plan_configure")

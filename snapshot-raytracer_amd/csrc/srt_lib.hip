@@ -452,6 +452,41 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void batch_kernel(const B
     }
 }
 
+// The same instances without a persistent loop: one workgroup per tile of
+// trace_lanes<F>() consecutive lanes of one instance, balanced by the
+// hardware dispatcher as in a single trace launch (no claims, no barriers).
+// The dispatcher hands workgroup b to XCD b mod 8; with `groups` ranges,
+// workgroup b takes position b / groups of range b mod groups.  Range x holds,
+// for every instance in turn, part x of its tiles (tiles [t x / groups,
+// t (x + 1) / groups) of its t), so every range carries the same mix of
+// instances and each part is a compact run of the instance's lane order.
+// parts[x * (n_inst + 1) + i] is the first position of instance i in range x;
+// parts[x * (n_inst + 1) + n_inst] is the range's length.  Workgroups past the
+// end of their range return.
+template <class F>
+__global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void batch_dispatch_kernel(
+    const BatchInst* __restrict__ inst, int n_inst, const int64_t* __restrict__ parts, int groups) {
+    constexpr int kLanes = trace_lanes<F>();
+    const int x = (int)(blockIdx.x % (unsigned)groups);
+    const int64_t k = blockIdx.x / (unsigned)groups;
+    const int64_t* pb = parts + (int64_t)x * (n_inst + 1);
+    if (k >= pb[n_inst]) return;
+    int a = 0, b = n_inst - 1;  // last instance whose part starts at or before k (it is not empty)
+    while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (pb[mid] <= k)
+            a = mid;
+        else
+            b = mid - 1;
+    }
+    const BatchInst& I = inst[a];
+    const int64_t tiles = (I.count + kLanes - 1) / kLanes;
+    const int64_t l = (tiles * x / groups + (k - pb[a])) * kLanes + threadIdx.x;
+    if (l >= I.count) return;
+    const int64_t slot = I.lane_slot ? (int64_t)I.lane_slot[l] : l;
+    trace_ray<F, kLanes>(I.G, I.nside, I.pix_begin + slot, slot, I.count, I.n_steps, I.obs, I.out, nullptr);
+}
+
 // ------------------------------------------------------------- adjoint ----
 // One ray per lane, FP64.  Forward pass storing the state every K steps
 // (checkpoints), the six maps in FP64, then, segment by segment from the
@@ -1278,6 +1313,12 @@ struct srt_plan {
     size_t order_bytes = 0;
     std::vector<const uint32_t*> item_map;     // per item: its map in order_buf, or null
     std::vector<uint64_t> item_gen;            // per item: its context's generation when the table was built
+    // dispatch by default: at 512^3 / Nside 1024 a one-item plan takes 464 ms against 521 ms persistent (the
+    // single trace launch: 463 ms), and 48 uneven snapshots 72 against 86 ms
+    int schedule = SRT_SCHEDULE_DISPATCH;
+    int64_t* d_parts = nullptr;  // batch_dispatch_kernel's range table, kMaxTileGroups * (n_inst + 1)
+    int dispatch_lanes = 0;      // lanes per dispatch tile the range table was built for
+    int64_t dispatch_blocks = 0;  // workgroups of a dispatch launch
     int tile_rays = 64;  // one 64-ray group per claim: in-flight rays of an XCD stay as compact as a trace launch's
     int blocks = 0;
     int64_t total_tiles = 0;
@@ -1372,9 +1413,30 @@ int plan_build_table(srt_plan* p) {
         tiles += (it.count + per_tile - 1) / per_tile;
     }
     p->total_tiles = tiles;
+    // range table of the dispatch schedule (batch_dispatch_kernel), for the
+    // workgroup size of the form it will launch
+    const int lanes = p->bicubic ? 64 : kBlock;
+    const int n = p->n_inst, g = p->groups;
+    std::vector<int64_t> parts((size_t)g * (n + 1));
+    int64_t longest = 0;
+    for (int x = 0; x < g; ++x) {
+        int64_t acc = 0;
+        for (int i = 0; i < n; ++i) {
+            parts[(size_t)x * (n + 1) + i] = acc;
+            const int64_t t = (p->items[i].count + lanes - 1) / lanes;
+            acc += t * (x + 1) / g - t * x / g;
+        }
+        parts[(size_t)x * (n + 1) + n] = acc;
+        longest = std::max(longest, acc);
+    }
+    if (longest * g > 0x7fffffff)
+        return fail(SRT_ERR_UNSUPPORTED, "plan: more than 2^31 - 1 workgroups for the dispatch schedule");
+    p->dispatch_lanes = lanes;
+    p->dispatch_blocks = longest * g;
     SRT_HIP(hipSetDevice(p->device));
     SRT_HIP(hipDeviceSynchronize());
     SRT_HIP(hipMemcpy(p->d_inst, h.data(), h.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+    SRT_HIP(hipMemcpy(p->d_parts, parts.data(), parts.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     p->item_gen.resize(p->n_inst);
     for (int i = 0; i < p->n_inst; ++i) p->item_gen[i] = p->items[i].ctx->generation;
     return SRT_OK;
@@ -1402,7 +1464,48 @@ void launch_batch(const srt_plan* p, hipStream_t st) {
                            p->total_tiles, p->tile_rays, p->groups, p->d_counter);
 }
 
+// batch_dispatch_kernel<F> for plan p (the dispatch schedule)
+template <class F>
+int launch_batch_dispatch(const srt_plan* p, hipStream_t st) {
+    constexpr int lanes = trace_lanes<F>();
+    constexpr unsigned lds = lds_bytes<F, lanes>();
+    if (lanes != p->dispatch_lanes)
+        return fail(SRT_ERR_DEVICE, "plan: range table built for another workgroup size");
+    if (p->dispatch_blocks == 0) {  // no rays: keep the launch sequence non-empty for graph capture
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, p->d_counter, p->groups);
+        return SRT_OK;
+    }
+    const dim3 grid((unsigned)p->dispatch_blocks);
+    if (p->pow2)
+        hipLaunchKernelGGL(batch_dispatch_kernel<srt::Pow2Grid<F>>, grid, dim3(lanes), lds, st, p->d_inst, p->n_inst,
+                           p->d_parts, p->groups);
+    else
+        hipLaunchKernelGGL(batch_dispatch_kernel<F>, grid, dim3(lanes), lds, st, p->d_inst, p->n_inst, p->d_parts,
+                           p->groups);
+    return SRT_OK;
+}
+
+int plan_enqueue_dispatch(srt_plan* p, hipStream_t st) {
+    int rc;
+    switch (p->precision) {
+        case SRT_PREC_FP64:
+            rc = launch_batch_dispatch<srt::FormFP64>(p, st);
+            break;
+        case SRT_PREC_MIXED:
+            rc = p->bicubic ? launch_batch_dispatch<srt::FormMixedBicubic>(p, st)
+                            : launch_batch_dispatch<srt::FormMixed>(p, st);
+            break;
+        default:
+            rc = launch_batch_dispatch<srt::FormRef>(p, st);
+            break;
+    }
+    if (rc != SRT_OK) return rc;
+    SRT_HIP(hipGetLastError());
+    return SRT_OK;
+}
+
 int plan_enqueue(srt_plan* p, hipStream_t st) {
+    if (p->schedule == SRT_SCHEDULE_DISPATCH) return plan_enqueue_dispatch(p, st);
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, p->d_counter, p->groups);
     switch (p->precision) {
         case SRT_PREC_FP64:
@@ -1461,6 +1564,7 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
     p->blocks = 2 * device_cus(device);
     hipError_t e = hipMalloc(&p->d_inst, sizeof(BatchInst) * n_items);
     if (e == hipSuccess) e = hipMalloc(&p->d_counter, kMaxTileGroups * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&p->d_parts, (size_t)kMaxTileGroups * (n_items + 1) * sizeof(int64_t));
     if (e == hipSuccess) e = hipEventCreate(&p->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&p->ev_stop);
     if (e != hipSuccess) {
@@ -1502,6 +1606,7 @@ int srt_plan_destroy(srt_plan* plan) {
     plan_drop_graph(plan);
     if (plan->d_inst) (void)hipFree(plan->d_inst);
     if (plan->d_counter) (void)hipFree(plan->d_counter);
+    if (plan->d_parts) (void)hipFree(plan->d_parts);
     if (plan->order_buf) (void)hipFree(plan->order_buf);
     if (plan->ev_start) (void)hipEventDestroy(plan->ev_start);
     if (plan->ev_stop) (void)hipEventDestroy(plan->ev_stop);
@@ -1520,6 +1625,23 @@ int srt_plan_configure(srt_plan* plan, int tile_rays, int blocks) {
     plan->tile_rays = tile_rays;
     plan->blocks = blocks;
     return plan_build_table(plan);
+}
+
+int srt_plan_set_schedule(srt_plan* plan, int schedule) {
+    if (!plan) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_set_schedule: null plan");
+    if (schedule != SRT_SCHEDULE_PERSISTENT && schedule != SRT_SCHEDULE_DISPATCH)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_set_schedule: unknown schedule");
+    SRT_HIP(hipSetDevice(plan->device));
+    SRT_HIP(hipDeviceSynchronize());
+    plan_drop_graph(plan);
+    plan->schedule = schedule;
+    return SRT_OK;
+}
+
+int srt_plan_get_schedule(srt_plan* plan, int* schedule) {
+    if (!plan || !schedule) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_get_schedule: null pointer");
+    *schedule = plan->schedule;
+    return SRT_OK;
 }
 
 int srt_plan_get_config(srt_plan* plan, int* tile_rays, int* blocks, int64_t* total_ray_steps) {
@@ -1583,8 +1705,10 @@ int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks)
     float best = 0.0f;
     int bt = plan->tile_rays, bb = plan->blocks, bg = plan->groups;
     bool have = false;
+    int rc = srt_plan_set_schedule(plan, SRT_SCHEDULE_PERSISTENT);
+    if (rc != SRT_OK) return rc;
     for (int g : group_counts) {
-        int rc = srt_plan_set_order(plan, plan->order, g);
+        rc = srt_plan_set_order(plan, plan->order, g);
         if (rc != SRT_OK) return rc;
         for (int t : tiles)
             for (int w : per_cu) {
@@ -1603,7 +1727,24 @@ int srt_plan_autotune(srt_plan* plan, void* stream, int* tile_rays, int* blocks)
                 }
             }
     }
-    int rc = srt_plan_set_order(plan, plan->order, bg);
+    // the dispatch schedule (no tile size or workgroup count to choose)
+    int bs = SRT_SCHEDULE_PERSISTENT, dg = bg;
+    rc = srt_plan_set_schedule(plan, SRT_SCHEDULE_DISPATCH);
+    for (int g : group_counts) {
+        if (rc == SRT_OK) rc = srt_plan_set_order(plan, plan->order, g);
+        if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);  // warm-up
+        float ms = 0.0f;
+        if (rc == SRT_OK) rc = srt_plan_launch(plan, 0, stream);
+        if (rc == SRT_OK) rc = srt_plan_last_ms(plan, &ms);
+        if (rc != SRT_OK) return rc;
+        if (ms < best) {
+            best = ms;
+            bs = SRT_SCHEDULE_DISPATCH;
+            dg = g;
+        }
+    }
+    rc = srt_plan_set_schedule(plan, bs);
+    if (rc == SRT_OK) rc = srt_plan_set_order(plan, plan->order, bs == SRT_SCHEDULE_DISPATCH ? dg : bg);
     if (rc == SRT_OK) rc = srt_plan_configure(plan, bt, bb);
     if (rc != SRT_OK) return rc;
     if (tile_rays) *tile_rays = bt;

@@ -51,6 +51,8 @@ def test_null_arguments_rejected_without_device_calls(srt):
     assert L.srt_plan_set_order(None, 1, 8) == 1
     assert L.srt_plan_autotune(None, None, None, None) == 1
     assert L.srt_plan_autotune_brick(None, None, None) == 1
+    assert L.srt_plan_set_schedule(None, 1) == 1
+    assert L.srt_plan_get_schedule(None, None) == 1
     assert b"null" in L.srt_last_error()
 
 

@@ -1,8 +1,9 @@
-"""Batched persistent launch (srt_plan_*): many snapshots with different grid
-sizes, observers, pixel ranges and step counts in one launch, direct or
-replayed from a HIP graph, with an autotuned tile size and grid.  Every
-instance must equal its own single-instance srt_trace bit for bit (the same
-per-ray code runs in both), on every replay."""
+"""Batched launch (srt_plan_*): many snapshots with different grid sizes,
+observers, pixel ranges and step counts in one launch, under the persistent
+(work-stealing) and the dispatch (one workgroup per tile) schedules, direct or
+replayed from a HIP graph, with an autotuned tile size, grid and schedule.
+Every instance must equal its own single-instance srt_trace bit for bit (the
+same per-ray code runs in both), on every replay."""
 import numpy as np
 import pytest
 
@@ -46,13 +47,19 @@ def check(items, ref):
         assert np.array_equal(it["out"].cpu().numpy(), r)
 
 
+SCHEDULES = {"persistent": 0, "dispatch": 1}
+
+
+@pytest.mark.parametrize("schedule", ["persistent", "dispatch"])
 @pytest.mark.parametrize("prec", ["mixed", "fp64"])
-def test_plan_direct_and_graph_replay(srt, instances, prec):
+def test_plan_direct_and_graph_replay(srt, instances, prec, schedule):
     import torch
-    print(f"[gpu] batch plan {prec}", flush=True)
+    print(f"[gpu] batch plan {prec}, {schedule}", flush=True)
     p = srt.PRECISIONS[prec]
     ref = expected(srt, instances, p)
     plan = srt.Plan(instances, precision=p)
+    plan.set_schedule(SCHEDULES[schedule])
+    assert plan.schedule() == SCHEDULES[schedule]
     plan.launch()
     torch.cuda.synchronize()
     check(instances, ref)
@@ -77,13 +84,15 @@ def test_plan_direct_and_graph_replay(srt, instances, prec):
     plan.close()
 
 
+@pytest.mark.parametrize("schedule", ["persistent", "dispatch"])
 @pytest.mark.parametrize("order,groups", [("linear", 1), ("linear", 8), ("nested", 1), ("nested", 3),
                                           ("nested", 64)])
-def test_plan_lane_order_and_tile_groups(srt, instances, order, groups):
+def test_plan_lane_order_and_tile_groups(srt, instances, order, groups, schedule):
     import torch
-    print(f"[gpu] batch plan order {order}, {groups} tile groups", flush=True)
+    print(f"[gpu] batch plan order {order}, {groups} tile groups, {schedule}", flush=True)
     ref = expected(srt, instances, srt.PREC_MIXED)
     plan = srt.Plan(instances, precision=srt.PREC_MIXED)
+    plan.set_schedule(SCHEDULES[schedule])
     plan.set_order(srt.ORDER_NESTED if order == "nested" else srt.ORDER_LINEAR, groups)
     s = torch.cuda.Stream()
     for graph in (False, True, True):
@@ -111,6 +120,7 @@ def test_plan_autotune(srt, instances):
     plan = srt.Plan(instances, precision=srt.PREC_MIXED)
     tile, blocks = plan.autotune()
     assert tile % 64 == 0 and blocks > 0
+    assert plan.schedule() in SCHEDULES.values()
     assert plan.config()[:2] == (tile, blocks)
     assert plan.config()[2] == sum(it["count"] * it["n_steps"] for it in instances)
     for it in instances:
@@ -131,17 +141,19 @@ def test_plan_errors(srt, instances):
         plan.configure(100, 4)  # not a multiple of 64
     with pytest.raises(srt.SrtError):
         plan.launch(graph=True)  # graph replay needs a non-default stream
+    with pytest.raises(srt.SrtError):
+        plan.set_schedule(2)
     plan.close()
 
 
-@pytest.mark.parametrize("tile,blocks", [(64, 1), (64, 5), (64, 512), (128, 37), (256, 2048)])
+@pytest.mark.parametrize("tile,blocks", [(64, 1), (64, 5), (64, 512), (128, 37), (256, 2048), (0, 0)])
 def test_plan_many_tiles_per_workgroup(srt, fields, tile, blocks):
     """One full-sky item and the 12 RING batches of a second sky as 13 items of
     one plan: thousands of 64-lane groups drawn from each workgroup's tile slot
     (one workgroup draws them all with blocks=1).  Every ray is traced exactly
     once: outputs start as NaN and must equal srt_trace's bit for bit."""
     import torch
-    print(f"[gpu] batch plan many tiles: tile {tile}, blocks {blocks}", flush=True)
+    print(f"[gpu] batch plan many tiles: tile {tile}, blocks {blocks} (0: dispatch schedule)", flush=True)
     dev = torch.device("cuda:0")
     phi, shift = fields.snapshot(32, 77)
     tr = srt.Tracer(0)
@@ -164,7 +176,11 @@ def test_plan_many_tiles_per_workgroup(srt, fields, tile, blocks):
                           out=torch.full((6 * per,), float("nan"), dtype=torch.float32, device=dev)))
     torch.cuda.synchronize()
     plan = srt.Plan(items, precision=srt.PREC_MIXED)
-    plan.configure(tile, blocks)
+    if tile:
+        plan.set_schedule(srt.SCHEDULE_PERSISTENT)
+        plan.configure(tile, blocks)
+    else:
+        plan.set_schedule(srt.SCHEDULE_DISPATCH)
     s = torch.cuda.Stream()
     for rep in range(3):
         for it in items:
@@ -200,18 +216,20 @@ def _items(tracers):
                  out=torch.zeros(6 * 68, dtype=torch.float32, device="cuda:0"))]
 
 
+@pytest.mark.parametrize("schedule", ["persistent", "dispatch"])
 @pytest.mark.parametrize("prec", ["mixed", "fp64"])
-def test_plan_follows_reupload_and_layout_changes(srt, fields, prec):
+def test_plan_follows_reupload_and_layout_changes(srt, fields, prec, schedule):
     """A plan built before a context gets a new snapshot (of another size) or
     another layout traces the context's current buffers at its next launch,
     direct or from a graph: the same outputs as fresh single traces."""
     import torch
-    print(f"[gpu] batch plan after re-upload / layout change, {prec}", flush=True)
+    print(f"[gpu] batch plan after re-upload / layout change, {prec}, {schedule}", flush=True)
     p = srt.PRECISIONS[prec]
     a = _tracer(srt, fields, 24, 301, srt.LAYOUT_BICUBIC)
     b = _tracer(srt, fields, 32, 302, srt.LAYOUT_BICUBIC)
     items = _items((a, b))
     plan = srt.Plan(items, precision=p)
+    plan.set_schedule(SCHEDULES[schedule])
     s = torch.cuda.Stream()
 
     def run_and_check(graph, only=(0, 1, 2)):

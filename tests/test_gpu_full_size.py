@@ -22,8 +22,9 @@ depend on size:
     snapshot 13 of 12.6 million pixels exceed 1e-5, up to 1.4e-4, each moving
     the oracle by exactly that amount under a shift of <= 3e-9 cells; random
     pixels move by <= 1 float ulp under 1e-8.);
-  * the 12 RING batches, as separate launches and as one persistent plan,
-    and LINEAR vs NESTED lane order: bit-identical to the full-sky launch.
+  * the 12 RING batches, as separate launches and as one plan (persistent and
+    dispatch schedules), and LINEAR vs NESTED lane order: bit-identical to the
+    full-sky launch.
 """
 import numpy as np
 import pytest
@@ -163,10 +164,14 @@ def test_full_size_batches_plan_and_lane_orders(srt, big, full_sky):
                           out=torch.full((6 * per,), float("nan"), dtype=torch.float32, device="cuda:0")))
     print("[gpu] full size: 12 batches as one plan", flush=True)
     plan = srt.Plan(items, precision=srt.PREC_MIXED)
-    plan.launch()
-    torch.cuda.synchronize()
-    for b, it in enumerate(items):
-        assert torch.equal(it["out"].view(6, per), mixed[:, b * per:(b + 1) * per]), b
+    for schedule in (srt.SCHEDULE_PERSISTENT, srt.SCHEDULE_DISPATCH):
+        plan.set_schedule(schedule)
+        for it in items:
+            it["out"].fill_(float("nan"))
+        plan.launch()
+        torch.cuda.synchronize()
+        for b, it in enumerate(items):
+            assert torch.equal(it["out"].view(6, per), mixed[:, b * per:(b + 1) * per]), (schedule, b)
     plan.close()
     print("[gpu] full size: linear lane order", flush=True)
     lin = torch.empty(6 * RAYS, dtype=torch.float32, device="cuda:0")
