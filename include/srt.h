@@ -226,7 +226,8 @@ typedef enum {
     /* One workgroup per tile of consecutive lanes of one item (64 lanes for
      * mixed plans over bicubic records, else 256), balanced by the hardware
      * dispatcher as a single srt_trace launch is; each tile range (see
-     * srt_plan_set_order) holds the same share of every item. */
+     * srt_plan_set_order) holds the same share of every item, items with
+     * more steps first. */
     SRT_SCHEDULE_DISPATCH = 1
 } srt_plan_schedule;
 

@@ -459,17 +459,19 @@ __global__ __launch_bounds__(kBlock, F::kWavesPerSimd) void batch_kernel(const B
 // workgroup b takes position b / groups of range b mod groups.  Range x holds,
 // for every instance in turn, part x of its tiles (tiles [t x / groups,
 // t (x + 1) / groups) of its t), so every range carries the same mix of
-// instances and each part is a compact run of the instance's lane order.
-// parts[x * (n_inst + 1) + i] is the first position of instance i in range x;
-// parts[x * (n_inst + 1) + n_inst] is the range's length.  Workgroups past the
-// end of their range return.
+// instances and each part is a compact run of the instance's lane order.  The
+// instances follow in `table[0 .. n_inst)`'s order (longest step count first,
+// so that the launch's last workgroups are short ones).  Then
+// table[n_inst + x * (n_inst + 1) + j] is the first position of the j-th
+// instance of that order in range x, and entry j = n_inst the range's length.
+// Workgroups past the end of their range return.
 template <class F>
 __global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void batch_dispatch_kernel(
-    const BatchInst* __restrict__ inst, int n_inst, const int64_t* __restrict__ parts, int groups) {
+    const BatchInst* __restrict__ inst, int n_inst, const int64_t* __restrict__ table, int groups) {
     constexpr int kLanes = trace_lanes<F>();
     const int x = (int)(blockIdx.x % (unsigned)groups);
     const int64_t k = blockIdx.x / (unsigned)groups;
-    const int64_t* pb = parts + (int64_t)x * (n_inst + 1);
+    const int64_t* pb = table + n_inst + (int64_t)x * (n_inst + 1);
     if (k >= pb[n_inst]) return;
     int a = 0, b = n_inst - 1;  // last instance whose part starts at or before k (it is not empty)
     while (a < b) {
@@ -479,7 +481,7 @@ __global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void batch_disp
         else
             b = mid - 1;
     }
-    const BatchInst& I = inst[a];
+    const BatchInst& I = inst[table[a]];
     const int64_t tiles = (I.count + kLanes - 1) / kLanes;
     const int64_t l = (tiles * x / groups + (k - pb[a])) * kLanes + threadIdx.x;
     if (l >= I.count) return;
@@ -1316,7 +1318,7 @@ struct srt_plan {
     // dispatch by default: at 512^3 / Nside 1024 a one-item plan takes 464 ms against 521 ms persistent (the
     // single trace launch: 463 ms), and 48 uneven snapshots 72 against 86 ms
     int schedule = SRT_SCHEDULE_DISPATCH;
-    int64_t* d_parts = nullptr;  // batch_dispatch_kernel's range table, kMaxTileGroups * (n_inst + 1)
+    int64_t* d_parts = nullptr;  // batch_dispatch_kernel's table: n_inst + kMaxTileGroups * (n_inst + 1)
     int dispatch_lanes = 0;      // lanes per dispatch tile the range table was built for
     int64_t dispatch_blocks = 0;  // workgroups of a dispatch launch
     int tile_rays = 64;  // one 64-ray group per claim: in-flight rays of an XCD stay as compact as a trace launch's
@@ -1417,16 +1419,20 @@ int plan_build_table(srt_plan* p) {
     // workgroup size of the form it will launch
     const int lanes = p->bicubic ? 64 : kBlock;
     const int n = p->n_inst, g = p->groups;
-    std::vector<int64_t> parts((size_t)g * (n + 1));
+    std::vector<int64_t> parts((size_t)n + (size_t)g * (n + 1));
+    for (int i = 0; i < n; ++i) parts[i] = i;
+    std::stable_sort(parts.begin(), parts.begin() + n,
+                     [&](int64_t a, int64_t b) { return p->items[a].n_steps > p->items[b].n_steps; });
     int64_t longest = 0;
     for (int x = 0; x < g; ++x) {
+        int64_t* pb = parts.data() + n + (size_t)x * (n + 1);
         int64_t acc = 0;
-        for (int i = 0; i < n; ++i) {
-            parts[(size_t)x * (n + 1) + i] = acc;
-            const int64_t t = (p->items[i].count + lanes - 1) / lanes;
+        for (int j = 0; j < n; ++j) {
+            pb[j] = acc;
+            const int64_t t = (p->items[parts[j]].count + lanes - 1) / lanes;
             acc += t * (x + 1) / g - t * x / g;
         }
-        parts[(size_t)x * (n + 1) + n] = acc;
+        pb[n] = acc;
         longest = std::max(longest, acc);
     }
     if (longest * g > 0x7fffffff)
@@ -1564,7 +1570,8 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
     p->blocks = 2 * device_cus(device);
     hipError_t e = hipMalloc(&p->d_inst, sizeof(BatchInst) * n_items);
     if (e == hipSuccess) e = hipMalloc(&p->d_counter, kMaxTileGroups * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&p->d_parts, (size_t)kMaxTileGroups * (n_items + 1) * sizeof(int64_t));
+    if (e == hipSuccess)
+        e = hipMalloc(&p->d_parts, ((size_t)n_items + (size_t)kMaxTileGroups * (n_items + 1)) * sizeof(int64_t));
     if (e == hipSuccess) e = hipEventCreate(&p->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&p->ev_stop);
     if (e != hipSuccess) {
