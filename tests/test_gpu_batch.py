@@ -51,7 +51,7 @@ SCHEDULES = {"persistent": 0, "dispatch": 1}
 
 
 @pytest.mark.parametrize("schedule", ["persistent", "dispatch"])
-@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+@pytest.mark.parametrize("prec", ["mixed", "fp64", "ref"])
 def test_plan_direct_and_graph_replay(srt, instances, prec, schedule):
     import torch
     print(f"[gpu] batch plan {prec}, {schedule}", flush=True)
