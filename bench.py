@@ -163,6 +163,18 @@ def workload_large(args, pkg, fields, dev):
         print(f"[bench] large bicubic_brick16, 12 batches in one dispatch-schedule launch: {ms:.0f} ms",
               file=sys.stderr, flush=True)
         plan.close()
+        # the full sky as one item, under both schedules (what the 12 batches cost beyond one item)
+        plan = pkg.Plan([dict(tracer=tr, nside=nside, pix_begin=0, count=rays, n_steps=n_steps, out=out)],
+                        precision=pkg.PREC_MIXED)
+        for sched_name, sched in (("persistent", pkg.SCHEDULE_PERSISTENT), ("dispatch", pkg.SCHEDULE_DISPATCH)):
+            plan.set_schedule(sched)
+            plan.launch(stream=stream)
+            ms = plan.last_ms()
+            res["bicubic_brick16"][f"plan_full_sky_{sched_name}"] = {
+                "kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0)}
+            print(f"[bench] large bicubic_brick16, full sky as one {sched_name} plan item: {ms:.0f} ms",
+                  file=sys.stderr, flush=True)
+        plan.close()
         del items, plan
         tr.set_layout(pkg.LAYOUT_BICUBIC, 1)
         run("bicubic_brick1", (("nested", pkg.ORDER_NESTED),))
