@@ -64,12 +64,22 @@ class SrtError(RuntimeError):
 
 
 def lib():
-    """Load libsrt.so (built by `make -C snapshot-raytracer_amd`)."""
+    """Load libsrt.so (built by `make -C snapshot-raytracer_amd`).
+
+    torch, when installed, is imported first: it carries its own HIP runtime,
+    and libsrt.so then binds to that one instead of loading the system's.  Two
+    HIP runtimes in one process do not share the GPU: whichever initialises
+    second finds no device.
+    """
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                                "or make -C snapshot-raytracer_amd")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.c_void_p
         i64 = ctypes.c_int64

@@ -89,3 +89,19 @@ def test_full_sky_assembly_masks(srt):
     lo = 3 * nside * nside
     assert np.all(full[:, :lo] == 0) and np.all(full[:, lo + nside * nside:] == 0)
     assert np.array_equal(full[:, lo:lo + nside * nside], maps)
+
+
+def test_one_hip_runtime_per_process():
+    """Loading the library through the package binds it to torch's HIP runtime
+    (torch is imported first), so the process maps one libamdhip64, whichever
+    of torch and the library touches the GPU first."""
+    import ast
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, {root!r}); import __graft_entry__; __graft_entry__.load_package(); "
+            "import snapshot_raytracer_amd as srt; srt.lib(); "
+            "print(sorted({{l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}}))").format(root=ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    libs = ast.literal_eval(out.stdout.strip().splitlines()[-1])
+    assert len(libs) == 1, libs
