@@ -33,7 +33,7 @@ import __graft_entry__  # noqa: E402
 FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12777.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
-HBM_BYTES_PER_LAUNCH = {"mixed": 27.42e9, "mixed_taps": 168.1e9, "fp64": 8.03e9}
+HBM_BYTES_PER_LAUNCH = {"mixed": 28.44e9, "mixed_taps": 110.2e9, "fp64": 8.06e9}
 # The reference algorithm's count (oracle with an operation-counting scalar).
 REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
 # MI355X published vector peaks (FP32 counts packed FP32; this build disables

@@ -1315,8 +1315,8 @@ struct srt_plan {
     size_t order_bytes = 0;
     std::vector<const uint32_t*> item_map;     // per item: its map in order_buf, or null
     std::vector<uint64_t> item_gen;            // per item: its context's generation when the table was built
-    // dispatch by default: at 512^3 / Nside 1024 a one-item plan takes 464 ms against 521 ms persistent (the
-    // single trace launch: 463 ms), and 48 uneven snapshots 72 against 86 ms
+    // dispatch by default: at 1024^3 / Nside 2048 / 512 steps a one-item plan takes 3.65 s against 3.77 s
+    // persistent (the single trace launch: 3.66 s), and 48 uneven snapshots 68.6 against 86.7 ms
     int schedule = SRT_SCHEDULE_DISPATCH;
     int64_t* d_parts = nullptr;  // batch_dispatch_kernel's table: n_inst + kMaxTileGroups * (n_inst + 1)
     int dispatch_lanes = 0;      // lanes per dispatch tile the range table was built for
