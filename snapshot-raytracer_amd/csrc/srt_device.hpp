@@ -555,16 +555,16 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
             static_assert(sizeof(RI) == 4, "bicubic records are FP32");
             // each field's 16 record loads and its evaluation stay together:
             // left free, the scheduler hoists later fields' loads into earlier
-            // fields, spills, and the kernel runs 10% slower at 2 wavefronts
-            // per SIMD
+            // fields and spills (472 instead of 450 ms at 512^3 / Nside 1024 /
+            // 256 steps)
             __builtin_amdgcn_sched_barrier(0);
             const Powers<RI> px = powers<RI>(a == 0 ? t1[0] : t0[0]);
             const Powers<RI> py = powers<RI>(a == 1 ? t1[1] : t0[1]);
             const Axis<RI> az = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
             return bicubic_column<RI>(G.rec[f] + record_offset(G, ix, iy, iz), px, py, az);
         } else {
-            // FP32 taps: the same fence as above (1.4% faster at 2 wavefronts
-            // per SIMD)
+            // FP32 taps: the same fence as above (581 instead of 738 ms on the
+            // same workload)
             if constexpr (!F::kGrid64) __builtin_amdgcn_sched_barrier(0);
             const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
             const size_t off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
