@@ -30,10 +30,10 @@ import __graft_entry__  # noqa: E402
 # from rocprofv3 counters SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP64
 # (per wavefront instruction, x 64 lanes) of one launch of this workload,
 # divided by its 3,221,225,472 ray-steps (profiles/trace_kernel_512cube_nside1024_counters.md).
-FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12777.0}
+FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12793.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
-HBM_BYTES_PER_LAUNCH = {"mixed": 28.44e9, "mixed_taps": 110.2e9, "fp64": 8.06e9}
+HBM_BYTES_PER_LAUNCH = {"mixed": 28.44e9, "mixed_taps": 108.4e9, "fp64": 7.87e9}
 # The reference algorithm's count (oracle with an operation-counting scalar).
 REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
 # MI355X published vector peaks (FP32 counts packed FP32; this build disables

@@ -308,9 +308,10 @@ struct Interp {
 // Tricubic interpolation as one fused contraction: 16 rows x 3 (x-pass),
 // 16 x 6 (y-pass), 4 x 10 (z-pass) = 328 multiply-adds.  `base` points at
 // tap (-1, -1, -1) of the stencil in the padded grid.  kSliceFence keeps
-// each z-slice's four row loads with its contraction (the REF kernel then
-// runs 929 instead of 947 ms at 512^3 / Nside 1024 / 256 steps; the mixed
-// padded-grid kernel is 1.5% faster without it).
+// each z-slice's four row loads with its contraction: at 512^3 / Nside 1024 /
+// 256 steps the FP64 kernel then runs 1625 instead of 1722 ms and the REF
+// kernel 929 instead of 947 ms; the mixed padded-grid kernel is 1.5% faster
+// without it.
 template <class R, bool kSliceFence = false, class GT>
 __device__ __forceinline__ Interp<R> tricubic(const GT* __restrict__ base, size_t pitch_y, size_t pitch_z,
                                               const Axis<R>& ax, const Axis<R>& ay, const Axis<R>& az) {
@@ -576,7 +577,7 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
             const Axis<RI> wy = cr_axis<RI>(a == 1 ? t1[1] : t0[1]);
             const Axis<RI> wz = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
             if constexpr (F::kGrid64)
-                return tricubic<RI>((f == 0 ? G.phi64 : G.b64[a]) + off, py, pz, wx, wy, wz);
+                return tricubic<RI, true>((f == 0 ? G.phi64 : G.b64[a]) + off, py, pz, wx, wy, wz);
             else
                 return tricubic<RI, F::kRefOffset>((f == 0 ? G.phi : G.b[a]) + off, py, pz, wx, wy, wz);
         }
