@@ -30,26 +30,37 @@ def build() -> None:
     subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
 
 
+def bind(path: str):
+    """Load an oracle library built from this directory's sources and declare its entry points.
+
+    lib() binds the default build; tests that compile a variant of
+    srt_oracle.hpp (a planted mistake, to show a check is sensitive to it)
+    bind that variant with this and pass it as ``library=``.
+    """
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i64 = ctypes.c_int64
+    L.orc_pix2zphi_ring.argtypes = [i64, P, i64, P, P, P]
+    L.orc_tricubic.argtypes = [ctypes.c_int, P, P, i64, i64, i64, P, i64, P]
+    L.orc_rhs.argtypes = [ctypes.c_int, P, P, P, P, P, i64, i64, i64, P, i64, P]
+    L.orc_trace.argtypes = [ctypes.c_int, P, P, P, P, P, i64, i64, i64, i64, P, i64,
+                            ctypes.c_int, P, P, P, ctypes.c_int, P]
+    L.orc_f32_to_f16_array.argtypes = [P, P, i64]
+    L.orc_f16_to_f32_array.argtypes = [P, P, i64]
+    L.orc_f16_check_range.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.orc_f16_check_range.restype = ctypes.c_uint64
+    L.orc_max_threads.restype = ctypes.c_int
+    if hasattr(L, "orc_count_ops"):
+        L.orc_count_ops.argtypes = [P, P, P, P, i64, i64, i64, i64, i64, ctypes.c_int, P, P]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
-        i64 = ctypes.c_int64
-        L.orc_pix2zphi_ring.argtypes = [i64, P, i64, P, P, P]
-        L.orc_tricubic.argtypes = [ctypes.c_int, P, P, i64, i64, i64, P, i64, P]
-        L.orc_rhs.argtypes = [ctypes.c_int, P, P, P, P, P, i64, i64, i64, P, i64, P]
-        L.orc_trace.argtypes = [ctypes.c_int, P, P, P, P, P, i64, i64, i64, i64, P, i64,
-                                ctypes.c_int, P, P, P, ctypes.c_int, P]
-        L.orc_count_ops.argtypes = [P, P, P, P, i64, i64, i64, i64, i64, ctypes.c_int, P, P]
-        L.orc_f32_to_f16_array.argtypes = [P, P, i64]
-        L.orc_f16_to_f32_array.argtypes = [P, P, i64]
-        L.orc_f16_check_range.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
-        L.orc_f16_check_range.restype = ctypes.c_uint64
-        L.orc_max_threads.restype = ctypes.c_int
-        _lib = L
+        _lib = bind(LIB_PATH)
     return _lib
 
 
@@ -122,8 +133,10 @@ def rhs(fields: Fields, y, form=FORM_FP64) -> np.ndarray:
 
 
 def trace(fields: Fields, nside: int, pix, n_steps: int, observer=None, form=FORM_FP64,
-          nthreads: int = 0, return_state: bool = False):
-    """Trace the rays of RING pixels `pix`; returns (maps (6, npix) float32, seconds[, state])."""
+          nthreads: int = 0, return_state: bool = False, library=None):
+    """Trace the rays of RING pixels `pix`; returns (maps (6, npix) float32, seconds[, state]).
+
+    library: an oracle build bound with bind() (default: lib())."""
     pix = np.ascontiguousarray(pix, dtype=np.int64)
     if observer is None:
         observer = (fields.nx * 0.5, fields.ny * 0.5, fields.nz * 0.5)
@@ -131,8 +144,8 @@ def trace(fields: Fields, nside: int, pix, n_steps: int, observer=None, form=FOR
     out = np.empty((6, pix.size), np.float32)
     state = np.empty((pix.size, 19), np.float64) if return_state else None
     sec = np.zeros(1, np.float64)
-    rc = lib().orc_trace(form, *fields.ptrs(), nside, _p(pix), pix.size, n_steps, _p(obs), _p(out),
-                         _p(state), nthreads, _p(sec))
+    rc = (library or lib()).orc_trace(form, *fields.ptrs(), nside, _p(pix), pix.size, n_steps, _p(obs),
+                                      _p(out), _p(state), nthreads, _p(sec))
     if rc:
         raise ValueError("invalid trace arguments")
     if return_state:
