@@ -266,15 +266,8 @@ def workload_batch(args, pkg, fields, dev):
     Times 48 separate launches, the plan under both schedules, then the autotuned plan."""
     import numpy as np
     import torch
-    rng = np.random.default_rng(7)
-    params = []
-    for i in range(48):
-        n = int(rng.choice([128, 192, 256, 384]))
-        nside = int(rng.choice([128, 256, 512]))
-        b = int(rng.integers(0, 12))
-        n_steps = int(rng.integers(64, n // 2 + 1))
-        obs = rng.uniform(0, n, 3)
-        params.append((n, nside, b, n_steps, obs))
+    params = [(q["n"], q["nside"], q["batch"], q["n_steps"], q["observer"])
+              for q in fields.uneven_batch_params()]
     rec_bytes = sum(4 * 64 * n * n * (n + 3) for n, *_ in params)
     taps_bytes = sum(4 * 4 * (n + 3) ** 3 for n, *_ in params)
     free, total = torch.cuda.mem_get_info()

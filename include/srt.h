@@ -91,8 +91,11 @@ typedef enum {
 /* Replaces the device selection and allocation bookkeeping of main()
  * (.cu:690-772).  device = HIP device ordinal.  Environment read here:
  * SRT_MAX_ALLOC_BYTES (largest single device allocation the context may make;
- * larger requests fail with SRT_ERR_OUT_OF_MEMORY, for exercising those paths)
- * and SRT_XCD_GROUPS (workgroup grouping of trace launches, default 8). */
+ * larger requests fail with SRT_ERR_OUT_OF_MEMORY, for exercising those paths),
+ * SRT_XCD_GROUPS (workgroup grouping of trace launches, default 8) and
+ * SRT_MAX_LAUNCH_RAYS (rays per trace launch, default and maximum 2^31: one
+ * launch may not hold 2^32 lanes, so a larger trace is split into several
+ * launches; smaller values exercise the split). */
 int srt_create(int device, srt_context** ctx);
 
 /* Frees every device buffer of the context (.cu:917-920, 989-998). */
@@ -136,6 +139,9 @@ int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_by
  *   Ray slot i traces RING pixel pix_list[i] if pix_list is not null (device
  *   pointer, count int64 values), else pixel pix_begin + i (.cu:708-717).
  *   observer: 3 doubles in grid units (host pointer); null = (nx, ny, nz) / 2 (.cu:560).
+ *   count: rays; the library cannot see the output buffers, so the caller
+ *        guarantees they hold count rays (a trace of 2^32 rays or more runs
+ *        as several launches).
  *   out: device pointer, 6 * count floats, map-major: out[m * count + i] with
  *        m = dA, ellipticity1, ellipticity2, rotation, deflection1, deflection2
  *        (.cu:577-582).
@@ -147,10 +153,19 @@ int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_by
  *   ray_order: srt_ray_order.  SRT_ORDER_NESTED other than over the full sky
  *        at a power-of-two Nside keeps a lane -> slot map (4 bytes per ray)
  *        and sort space in the context: overlapping such calls on one
- *        context must share a stream. */
+ *        context must share a stream.  Such a sorted order covers at most
+ *        2^31 - 1 rays; a larger request runs in SRT_ORDER_LINEAR order
+ *        (srt_effective_order), with the same maps. */
 int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
               int n_steps, const double* observer, int precision, int ray_order, float* out,
               double* final_state, void* stream);
+
+/* The lane order srt_trace runs for a request (no counterpart in the
+ * reference): *effective = SRT_ORDER_NESTED or SRT_ORDER_LINEAR for a trace
+ * of `count` rays from pix_begin (or of a pixel list when has_pix_list != 0)
+ * asked for in `ray_order`.  Host-only: needs no device or context. */
+int srt_effective_order(int64_t nside, int64_t pix_begin, int64_t count, int has_pix_list, int ray_order,
+                        int* effective);
 
 /* Synchronous host-memory variant of srt_trace for callers without device
  * buffers (the CLI): pix_list, out (6 * count floats) and final_state are host

@@ -45,7 +45,7 @@ EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_u
            "srt_get_layout", "srt_trace",
            "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_set_order", "srt_plan_configure",
            "srt_plan_get_config", "srt_plan_set_schedule", "srt_plan_get_schedule", "srt_plan_autotune", "srt_plan_autotune_brick", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
-           "srt_grid_info", "srt_version")
+           "srt_grid_info", "srt_version", "srt_effective_order")
 
 
 class BatchItem(ctypes.Structure):
@@ -109,6 +109,7 @@ def lib():
         L.srt_plan_last_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.srt_last_trace_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.srt_grid_info.argtypes = [P, P, P, P, P]
+        L.srt_effective_order.argtypes = [i64, i64, i64, i32, i32, ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -135,6 +136,15 @@ def batch_range(nside: int, batch: int | None) -> tuple[int, int]:
     return start, nside * nside
 
 
+def effective_order(nside: int, pix_begin: int, count: int, has_pix_list: bool = False,
+                    order: int = ORDER_NESTED) -> int:
+    """The lane order a trace runs for a request (srt_effective_order; host only, no GPU)."""
+    v = ctypes.c_int()
+    _check(lib().srt_effective_order(nside, pix_begin, count, 1 if has_pix_list else 0, order, ctypes.byref(v)),
+           "srt_effective_order")
+    return v.value
+
+
 def _ptr(t) -> ctypes.c_void_p:
     """Address of a numpy array or torch tensor (host or device)."""
     if t is None:
@@ -152,6 +162,7 @@ class Tracer:
         _check(lib().srt_create(device, ctypes.byref(self._ctx)), "srt_create")
         self.device = device
         self.n = None
+        self.uploads = 0  # snapshots uploaded so far (ray_maps uses it to notice a replaced snapshot)
 
     def close(self) -> None:
         if self._ctx:
@@ -183,6 +194,7 @@ class Tracer:
         nz, ny, nx = phi.shape
         if tuple(shift_xyz.shape) != (nz, ny, nx, 3):
             raise ValueError("shift must be (nz, ny, nx, 3) interleaved")
+        self.uploads += 1
         _check(lib().srt_upload(self._ctx, _ptr(phi), _ptr(shift_xyz), nx, ny, nz, on_dev, shift_format),
                "srt_upload")
         self.n = nx
@@ -277,6 +289,8 @@ class Plan:
 
     items: list of dicts with keys tracer (Tracer), nside, pix_begin, count,
     n_steps, observer (optional) and out (float32 CUDA tensor of 6 * count).
+    The plan holds references to the items' tracers and outputs, so their
+    contexts stay alive while the plan does.
     """
 
     def __init__(self, items, precision: int = PREC_MIXED):
@@ -296,6 +310,7 @@ class Plan:
                     b.observer[k] = float(obs[k])
             b.out = it["out"].data_ptr()
             self._keep.append(it["out"])
+            self._keep.append(it["tracer"])
         self._plan = ctypes.c_void_p()
         _check(lib().srt_plan_create(arr, len(items), precision, ctypes.byref(self._plan)), "srt_plan_create")
 
@@ -373,7 +388,11 @@ def ray_maps(phi, shift_xyz, nside: int, n_steps: int, *, tracer: Tracer | None 
 
     phi (n, n, n) and shift_xyz (n, n, n, 3) are CUDA tensors (stored on the
     device as float32, the precision the grids have in the ray tracer); the
-    backward pass runs the adjoint kernel (srt_trace_adjoint).
+    backward pass runs the adjoint kernel (srt_trace_adjoint).  With `tracer`
+    given, the forward pass uploads phi and shift into it, replacing its
+    snapshot; the backward pass uploads them again if the tracer holds another
+    snapshot by then (e.g. after a second ray_maps call on the same tracer),
+    so each output's gradient is taken at its own snapshot.
     """
     import torch
 
@@ -381,15 +400,21 @@ def ray_maps(phi, shift_xyz, nside: int, n_steps: int, *, tracer: Tracer | None 
         @staticmethod
         def forward(ctx, phi_t, shift_t):
             tr = tracer or Tracer(phi_t.device.index or 0)
-            tr.upload(phi_t.detach().float().contiguous(), shift_t.detach().float().contiguous())
+            grids = (phi_t.detach().float().contiguous(), shift_t.detach().float().contiguous())
+            tr.upload(*grids)
             maps, _, _ = tr.trace_adjoint(nside, n_steps, None, checkpoint_every=checkpoint_every,
                                           pix_begin=pix_begin, count=count, pix_list=pix_list, observer=observer)
             ctx.tr = tr
+            ctx.grids = grids
+            ctx.upload = tr.uploads
             ctx.dtypes = (phi_t.dtype, shift_t.dtype)
             return maps
 
         @staticmethod
         def backward(ctx, grad_maps):
+            if ctx.tr.uploads != ctx.upload:  # the tracer was given another snapshot since forward
+                ctx.tr.upload(*ctx.grids)
+                ctx.upload = ctx.tr.uploads
             _, gphi, gshift = ctx.tr.trace_adjoint(nside, n_steps, grad_maps, checkpoint_every=checkpoint_every,
                                                    pix_begin=pix_begin, count=count, pix_list=pix_list,
                                                    observer=observer)

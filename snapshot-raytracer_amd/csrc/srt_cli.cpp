@@ -167,12 +167,16 @@ int main(int argc, char* argv[]) {
         }
     }
 
-    std::cout << "Launching raytracing kernel (" << prec_s << ")..." << std::endl;
+    // compact sky patches per wavefront (a batch's rays are sorted on the device,
+    // up to 2^31 - 1 of them; beyond that the library keeps LINEAR order); the
+    // maps do not depend on the lane order
+    const int order = SRT_ORDER_NESTED;
+    int lane_order = order;
+    (void)srt_effective_order(nside, istart, count, 0, order, &lane_order);
+    std::cout << "Launching raytracing kernel (" << prec_s << ", "
+              << (lane_order == SRT_ORDER_NESTED ? "nested" : "linear") << " lane order)..." << std::endl;
     std::vector<float> maps((size_t)(6 * count));
     const auto t0 = std::chrono::steady_clock::now();
-    // compact sky patches per wavefront (a batch's rays are sorted on the device);
-    // the maps do not depend on the lane order
-    const int order = SRT_ORDER_NESTED;
     rc = srt_trace_host(ctx, nside, istart, count, nullptr, (int)n_steps, nullptr, precision, order, maps.data(),
                         nullptr);
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -163,6 +163,8 @@ struct TraceArgs {
     double* state;
     int xcd_groups;             // see tile_of_block
     const uint32_t* lane_slot;  // sorted lane -> slot map (srt::order_lanes), or null
+    int64_t lane_base;          // first lane of this launch (a trace is split into launches of
+                                // at most srt_context::max_launch_rays lanes)
 };
 
 // Ray tile traced by workgroup `b` of `nb`.  The dispatcher hands workgroups
@@ -360,7 +362,8 @@ constexpr int trace_lanes() {
 template <class F>
 __global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void trace_kernel(TraceArgs A) {
     constexpr int kLanes = trace_lanes<F>();
-    const int64_t lane_ray = tile_of_block(blockIdx.x, gridDim.x, A.xcd_groups) * kLanes + threadIdx.x;
+    const int64_t lane_ray =
+        A.lane_base + tile_of_block(blockIdx.x, gridDim.x, A.xcd_groups) * kLanes + threadIdx.x;
     if (lane_ray >= A.count) return;
     int64_t pix, slot;
     const bool nested_sky = A.order == SRT_ORDER_NESTED && !A.lane_slot;
@@ -779,16 +782,23 @@ constexpr unsigned lds_bytes() {
     return (unsigned)(kLanes * (19 * (sizeof(double) + sizeof(RA)) + (sizeof(RA) == 4 ? 10 * sizeof(RA) : 0)));
 }
 
-// trace_kernel<F> over A.count lanes, specialised for a power-of-two grid edge
+// trace_kernel<F> over A.count lanes, specialised for a power-of-two grid
+// edge, in launches of at most `max_lanes` lanes (a multiple of 256): one
+// launch may not hold 2^32 or more lanes (grid size times workgroup size)
 template <class F>
-void launch_trace(bool pow2, hipStream_t st, const TraceArgs& A) {
+void launch_trace(bool pow2, hipStream_t st, const TraceArgs& A_in, int64_t max_lanes) {
     constexpr int lanes = trace_lanes<F>();
     constexpr unsigned lds = lds_bytes<F, lanes>();
-    const unsigned blocks = (unsigned)((A.count + lanes - 1) / lanes);
-    if (pow2)
-        hipLaunchKernelGGL(trace_kernel<srt::Pow2Grid<F>>, dim3(blocks), dim3(lanes), lds, st, A);
-    else
-        hipLaunchKernelGGL(trace_kernel<F>, dim3(blocks), dim3(lanes), lds, st, A);
+    TraceArgs A = A_in;
+    for (int64_t base = 0; base < A.count; base += max_lanes) {
+        A.lane_base = base;
+        const int64_t n = std::min(max_lanes, A.count - base);
+        const unsigned blocks = (unsigned)((n + lanes - 1) / lanes);
+        if (pow2)
+            hipLaunchKernelGGL(trace_kernel<srt::Pow2Grid<F>>, dim3(blocks), dim3(lanes), lds, st, A);
+        else
+            hipLaunchKernelGGL(trace_kernel<F>, dim3(blocks), dim3(lanes), lds, st, A);
+    }
 }
 
 }  // namespace
@@ -814,6 +824,11 @@ struct srt_context {
     // workgroup -> tile grouping of trace launches (tile_of_block); from
     // SRT_XCD_GROUPS at context creation, default 8 (the XCDs of an MI355X)
     int xcd_groups = 8;
+    // lanes per trace launch (a multiple of 256): a launch may not hold 2^32
+    // or more lanes, so larger traces are split; from SRT_MAX_LAUNCH_RAYS at
+    // context creation (rounded down to a multiple of 256), so that tests can
+    // exercise the split with small traces
+    int64_t max_launch_rays = int64_t(1) << 31;
     // sorted lane order of partial-sky NESTED traces: lane -> slot map and the
     // sort's work space, grown on demand
     void* order_buf = nullptr;
@@ -920,6 +935,22 @@ void set_records(const srt_context* ctx, srt::Grids& G) {
     G.rec_nb = brick_columns(ctx->n, ctx->brick_shift);
 }
 
+// Lane order a trace of `count` rays runs in for a requested order: NESTED over
+// the full sky at a power-of-two Nside needs no sort (lane = NESTED pixel);
+// any other NESTED request sorts the rays, which covers at most
+// srt::kMaxOrderedRays rays, and beyond that the trace runs in LINEAR order
+// (the maps do not depend on the order).  `sorted` says whether a sort is run.
+int effective_order(int64_t nside, int64_t pix_begin, int64_t count, bool has_list, int requested, bool* sorted) {
+    const bool sky = !has_list && pix_begin == 0 && count == 12 * nside * nside && (nside & (nside - 1)) == 0;
+    const bool sort = requested == SRT_ORDER_NESTED && count > 0 && !sky;
+    if (sort && count > srt::kMaxOrderedRays) {
+        if (sorted) *sorted = false;
+        return SRT_ORDER_LINEAR;
+    }
+    if (sorted) *sorted = sort;
+    return requested;
+}
+
 }  // namespace
 
 extern "C" {
@@ -941,6 +972,17 @@ const char* srt_strerror(int status) {
 
 const char* srt_last_error(void) { return g_last_error.c_str(); }
 
+int srt_effective_order(int64_t nside, int64_t pix_begin, int64_t count, int has_pix_list, int ray_order,
+                        int* effective) {
+    if (!effective) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_effective_order: null output");
+    if (nside <= 0 || nside > (1 << 20) || count < 0)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_effective_order: nside or count out of range");
+    if (ray_order != SRT_ORDER_NESTED && ray_order != SRT_ORDER_LINEAR)
+        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_effective_order: unknown ray order");
+    *effective = effective_order(nside, pix_begin, count, has_pix_list != 0, ray_order, nullptr);
+    return SRT_OK;
+}
+
 int srt_create(int device, srt_context** ctx) {
     if (!ctx) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_create: ctx is null");
     *ctx = nullptr;
@@ -959,6 +1001,10 @@ int srt_create(int device, srt_context** ctx) {
     if (const char* m = std::getenv("SRT_MAX_ALLOC_BYTES")) {
         const long long v = std::atoll(m);
         if (v > 0) c->max_alloc = (size_t)v;
+    }
+    if (const char* m = std::getenv("SRT_MAX_LAUNCH_RAYS")) {
+        const long long v = std::atoll(m) / 256 * 256;
+        if (v >= 256) c->max_launch_rays = std::min<int64_t>(v, c->max_launch_rays);
     }
     hipError_t e1 = hipEventCreate(&c->ev_start);
     hipError_t e2 = hipEventCreate(&c->ev_stop);
@@ -1113,19 +1159,15 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown ray order");
     if (precision != SRT_PREC_FP64 && precision != SRT_PREC_MIXED && precision != SRT_PREC_REF)
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
-    // one workgroup per 64 lanes at most: the grid must stay within 2^31 - 1 workgroups
-    if ((count + 63) / 64 > (int64_t)0x7fffffff)
-        return fail(SRT_ERR_UNSUPPORTED, "srt_trace: more rays than one launch holds (2^31 - 1 wavefronts)");
     SRT_HIP(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     // NESTED over the full sky at power-of-two Nside: lanes are NESTED pixels;
-    // any other NESTED request sorts the rays by srt::ring_order_key
-    const bool sorted = ray_order == SRT_ORDER_NESTED && count > 0 &&
-                        (pix_list || pix_begin != 0 || count != npix || (nside & (nside - 1)) != 0);
+    // any other NESTED request sorts the rays by srt::ring_order_key (LINEAR
+    // beyond the sort's ray limit)
+    bool sorted = false;
+    ray_order = effective_order(nside, pix_begin, count, pix_list != nullptr, ray_order, &sorted);
     uint32_t* lane_slot = nullptr;
     if (sorted) {
-        if (count > srt::kMaxOrderedRays)
-            return fail(SRT_ERR_UNSUPPORTED, "srt_trace: NESTED lane order over more than 2^31 - 1 rays");
         const size_t map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
         const size_t work = srt::order_work_bytes(nside, count);
         if (!work) return fail(SRT_ERR_DEVICE, "srt_trace: lane-order work size");
@@ -1163,6 +1205,7 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     A.state = final_state;
     A.xcd_groups = ctx->xcd_groups;
     A.lane_slot = lane_slot;
+    A.lane_base = 0;
     SRT_HIP(hipEventRecord(ctx->ev_start, st));
     if (lane_slot) {  // the sort is part of the timed trace
         const size_t map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
@@ -1173,16 +1216,16 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
         const bool pow2 = (ctx->n & (ctx->n - 1)) == 0;
         switch (precision) {
             case SRT_PREC_FP64:
-                launch_trace<srt::FormFP64>(pow2, st, A);
+                launch_trace<srt::FormFP64>(pow2, st, A, ctx->max_launch_rays);
                 break;
             case SRT_PREC_MIXED:
                 if (ctx->rec)
-                    launch_trace<srt::FormMixedBicubic>(pow2, st, A);
+                    launch_trace<srt::FormMixedBicubic>(pow2, st, A, ctx->max_launch_rays);
                 else
-                    launch_trace<srt::FormMixed>(pow2, st, A);
+                    launch_trace<srt::FormMixed>(pow2, st, A, ctx->max_launch_rays);
                 break;
             case SRT_PREC_REF:
-                launch_trace<srt::FormRef>(pow2, st, A);
+                launch_trace<srt::FormRef>(pow2, st, A, ctx->max_launch_rays);
                 break;
             default:
                 return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
@@ -1209,8 +1252,9 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     if (map_grad && (!grad_phi || !grad_shift_xyz))
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: gradient outputs are null");
     if (!map_grad && !maps64) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: nothing to compute");
-    if ((count + kAdjBlock - 1) / kAdjBlock > (int64_t)0x7fffffff)
-        return fail(SRT_ERR_UNSUPPORTED, "srt_trace_adjoint: more rays than one launch holds");
+    // one launch, whose lanes (grid size times workgroup size) must stay below 2^32
+    if ((count + kAdjBlock - 1) / kAdjBlock * kAdjBlock >= (int64_t(1) << 32))
+        return fail(SRT_ERR_UNSUPPORTED, "srt_trace_adjoint: more rays than one launch holds (2^32 lanes)");
     SRT_HIP(hipSetDevice(ctx->device));
     const int K = std::min(checkpoint_every, n_steps);
     const int nseg = (n_steps + K - 1) / K;
@@ -1435,8 +1479,8 @@ int plan_build_table(srt_plan* p) {
         pb[n] = acc;
         longest = std::max(longest, acc);
     }
-    if (longest * g > 0x7fffffff)
-        return fail(SRT_ERR_UNSUPPORTED, "plan: more than 2^31 - 1 workgroups for the dispatch schedule");
+    if (longest * g * lanes >= (int64_t(1) << 32))
+        return fail(SRT_ERR_UNSUPPORTED, "plan: the dispatch schedule's launch would hold 2^32 or more lanes");
     p->dispatch_lanes = lanes;
     p->dispatch_blocks = longest * g;
     SRT_HIP(hipSetDevice(p->device));

@@ -102,3 +102,26 @@ def snapshot(n: int, seed: int, phi_rms: float = 1e-5, shift_rms: float = 1e-7, 
 def interleave_shift(shift: torch.Tensor) -> torch.Tensor:
     """(3, n, n, n) planar -> (n, n, n, 3) interleaved, the HDF5 H5T_ARRAY{3} layout."""
     return shift.permute(1, 2, 3, 0).contiguous()
+
+
+def uneven_batch_params(count: int = 48, rng_seed: int = 7):
+    """Parameters of the batched uneven workload: `count` snapshot/observer instances.
+
+    Instance i (numpy default_rng(rng_seed), drawn in this order): grid edge n
+    from {128, 192, 256, 384}, Nside from {128, 256, 512}, one RING batch b in
+    [0, 12), n_steps uniform in [64, n / 2], observer uniform in [0, n)^3;
+    snapshot seed 1000 + 4 i.  Returns a list of dicts with keys n, nside,
+    batch, n_steps, observer (3 floats), seed.
+    """
+    import numpy as np
+
+    rng = np.random.default_rng(rng_seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice([128, 192, 256, 384]))
+        nside = int(rng.choice([128, 256, 512]))
+        b = int(rng.integers(0, 12))
+        n_steps = int(rng.integers(64, n // 2 + 1))
+        obs = rng.uniform(0, n, 3)
+        out.append(dict(n=n, nside=nside, batch=b, n_steps=n_steps, observer=obs, seed=1000 + 4 * i))
+    return out

@@ -35,3 +35,17 @@ def oracle():
 def fields():
     from snapshot_raytracer_amd import fields as f
     return f
+
+
+@pytest.fixture(scope="session")
+def progress(request):
+    """progress(msg): a line on the terminal now, even while pytest captures output (long GPU steps)."""
+    rep = request.config.pluginmanager.get_plugin("terminalreporter")
+
+    def emit(msg: str) -> None:
+        if rep is not None:
+            rep.write_line(msg)
+            rep.flush()
+        else:
+            print(msg, flush=True)
+    return emit

@@ -105,3 +105,24 @@ def test_one_hip_runtime_per_process():
     assert out.returncode == 0, out.stderr
     libs = ast.literal_eval(out.stdout.strip().splitlines()[-1])
     assert len(libs) == 1, libs
+
+
+def test_effective_lane_order_host_only(srt):
+    """NESTED traces sort their rays unless they cover the full sky at a power-of-two Nside; a sort
+    covers at most 2^31 - 1 rays, and larger requests (the CLI's batches at Nside >= 46341, full skies
+    at non-power-of-two Nside >= 13378) run in LINEAR order instead of failing."""
+    N, L = srt.ORDER_NESTED, srt.ORDER_LINEAR
+    big = 46341  # one RING batch of Nside^2 > 2^31 - 1 rays
+    assert big * big > 2 ** 31 - 1 > (big - 1) ** 2
+    assert srt.effective_order(big, 0, big * big) == L
+    assert srt.effective_order(big - 1, 0, (big - 1) ** 2) == N
+    assert srt.effective_order(13378, 0, 12 * 13378 ** 2) == L  # full sky, not a power of two
+    assert srt.effective_order(13377, 0, 12 * 13377 ** 2) == N
+    assert srt.effective_order(1 << 15, 0, 12 << 30) == N  # full sky at 2^15: lanes are NESTED pixels
+    assert srt.effective_order(1 << 15, 0, 12 << 30, has_pix_list=True) == L
+    assert srt.effective_order(1024, 5, 100, order=L) == L
+    assert srt.effective_order(1024, 5, 100, has_pix_list=True) == N
+    with pytest.raises(srt.SrtError):
+        srt.effective_order(0, 0, 1)
+    with pytest.raises(srt.SrtError):
+        srt.effective_order(4, 0, 1, order=7)

@@ -212,3 +212,49 @@ def test_adjoint_pixel_lists_partial_waves_and_invalid_pixels(srt, setup):
     assert np.array_equal(m3[:, valid], m1) and np.array_equal(m2[:, ::-1], m1)
     for a, b in ((gp2, gp1), (gs2, gs1), (gp3, gp1), (gs3, gs1)):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_autograd_two_snapshots_on_one_tracer(srt, fields, setup):
+    """Two ray_maps calls on one tracer (two snapshots) and one backward through a loss over both:
+    each output's gradient is taken at its own snapshot, equal to separate adjoint calls."""
+    import torch
+    tr, phi, shift, target = setup
+    dev = torch.device("cuda:0")
+    phi2, shift2 = fields.snapshot(N, 36)
+    grids = []
+    for p_np, s_np in ((phi, shift), (phi2.numpy(), shift2.numpy())):
+        p = torch.from_numpy(p_np).to(dev).double().requires_grad_(True)
+        s = torch.from_numpy(np.ascontiguousarray(np.moveaxis(s_np, 0, -1))).to(dev).double().requires_grad_(True)
+        grids.append((p, s))
+    m1 = srt.ray_maps(*grids[0], NSIDE, STEPS, tracer=tr, checkpoint_every=K)
+    m2 = srt.ray_maps(*grids[1], NSIDE, STEPS, tracer=tr, checkpoint_every=K)
+    loss = ((m1[[1, 2, 0]] - target) ** 2).sum() + ((m2[[1, 2, 0]] - target) ** 2).sum()
+    loss.backward()
+    for (p, s), m in zip(grids, (m1, m2)):
+        tr.upload(p.detach().float().contiguous(), s.detach().float().contiguous())
+        _, g = loss_and_grad(m.detach(), target)
+        _, gphi, gshift = tr.trace_adjoint(NSIDE, STEPS, g, checkpoint_every=K)
+        np.testing.assert_allclose(p.grad.cpu().numpy(), gphi.cpu().numpy(), rtol=1e-9,
+                                   atol=1e-12 * float(gphi.abs().max()))
+        np.testing.assert_allclose(s.grad.cpu().numpy(), gshift.cpu().numpy(), rtol=1e-9,
+                                   atol=1e-12 * float(gshift.abs().max()))
+    assert not torch.allclose(grids[0][0].grad, grids[1][0].grad)
+
+
+def test_adjoint_rejects_launches_of_2_to_the_32_lanes(srt, setup):
+    """A ray count whose single adjoint launch would hold 2^32 lanes is refused before any device
+    work or allocation; the output is left untouched."""
+    import ctypes
+    import torch
+    tr, phi, shift, _ = setup
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
+    L = srt.lib()
+    maps64 = torch.full((16,), 7.0, dtype=torch.float64, device="cuda:0")
+    nside = 1 << 15  # 12 * 2^30 pixels
+    for count in (1 << 32, (1 << 32) - 63):
+        rc = L.srt_trace_adjoint(tr._ctx, nside, 0, count, None, 4, None, 4, None,
+                                 ctypes.c_void_p(maps64.data_ptr()), None, None, None)
+        assert rc == 6, (count, rc)
+        assert b"2^32" in L.srt_last_error()
+    torch.cuda.synchronize()
+    assert bool((maps64 == 7.0).all())

@@ -3,7 +3,10 @@ observers, pixel ranges and step counts in one launch, under the persistent
 (work-stealing) and the dispatch (one workgroup per tile) schedules, direct or
 replayed from a HIP graph, with an autotuned tile size, grid and schedule.
 Every instance must equal its own single-instance srt_trace bit for bit (the
-same per-ray code runs in both), on every replay."""
+same per-ray code runs in both), on every replay, and the plan's maps must
+match the host oracle: FP64 plans within 2 float ulps of the FP64 oracle,
+mixed plans within dA 1e-5 / shear 1e-4 of it, REF plans within dA 1e-6 /
+2e-5 of each map's rms of the oracle's REF form."""
 import numpy as np
 import pytest
 
@@ -26,7 +29,8 @@ def instances(srt, fields):
         n_steps = int(rng.integers(4, n // 2 + 1))
         obs = rng.uniform(0, n, 3)
         out.append(dict(tracer=tr, nside=nside, pix_begin=lo, count=cnt, n_steps=n_steps, observer=obs,
-                        out=torch.zeros(6 * cnt, dtype=torch.float32, device=dev)))
+                        out=torch.zeros(6 * cnt, dtype=torch.float32, device=dev),
+                        host=(phi.numpy(), shift.numpy())))
     return out
 
 
@@ -47,12 +51,34 @@ def check(items, ref):
         assert np.array_equal(it["out"].cpu().numpy(), r)
 
 
+def check_oracle(oracle, items, prec):
+    """Each item's plan output against the host oracle over its whole pixel range."""
+    form = oracle.FORM_REF if prec == "ref" else oracle.FORM_FP64
+    for it in items:
+        pix = np.arange(it["pix_begin"], it["pix_begin"] + it["count"])
+        ref, _ = oracle.trace(oracle.Fields(*it["host"]), it["nside"], pix, it["n_steps"],
+                              observer=it["observer"], form=form)
+        got = it["out"].view(6, it["count"]).cpu().numpy()
+        ref64, got64 = ref.astype(np.float64), got.astype(np.float64)
+        da = np.max(np.abs(got64[0] - ref64[0]) / np.abs(ref64[0]))
+        if prec == "fp64":
+            np.testing.assert_allclose(got, ref, rtol=2.5e-7, atol=1e-30)
+        elif prec == "mixed":
+            de2 = (got64[1] - ref64[1]) ** 2 + (got64[2] - ref64[2]) ** 2
+            assert da <= 1e-5
+            assert np.sqrt(np.mean(de2) / np.mean(ref64[1] ** 2 + ref64[2] ** 2)) <= 1e-4
+        else:
+            assert da < 1e-6
+            for m in range(1, 6):
+                assert np.abs(got64[m] - ref64[m]).max() / np.sqrt(np.mean(ref64[m] ** 2)) < 2e-5, m
+
+
 SCHEDULES = {"persistent": 0, "dispatch": 1}
 
 
 @pytest.mark.parametrize("schedule", ["persistent", "dispatch"])
 @pytest.mark.parametrize("prec", ["mixed", "fp64", "ref"])
-def test_plan_direct_and_graph_replay(srt, instances, prec, schedule):
+def test_plan_direct_and_graph_replay(srt, oracle, instances, prec, schedule):
     import torch
     print(f"[gpu] batch plan {prec}, {schedule}", flush=True)
     p = srt.PRECISIONS[prec]
@@ -63,6 +89,7 @@ def test_plan_direct_and_graph_replay(srt, instances, prec, schedule):
     plan.launch()
     torch.cuda.synchronize()
     check(instances, ref)
+    check_oracle(oracle, instances, prec)
     s = torch.cuda.Stream()
     for rep in range(3):
         for it in instances:  # eager work between replays
@@ -289,3 +316,37 @@ def test_plan_autotune_brick(srt, fields):
     fp64.close()
     a.close()
     b.close()
+
+
+def test_plan_keeps_its_tracers_alive(srt, fields):
+    """A plan built from tracers that the caller no longer references still traces their snapshots:
+    the plan holds the tracers, so their contexts (and device grids) are not destroyed under it."""
+    import gc
+    import torch
+    print("[gpu] batch plan with out-of-scope tracers", flush=True)
+
+    def make():
+        items, refs = [], []
+        for i, n in enumerate((16, 24)):
+            phi, shift = fields.snapshot(n, 321 + i)
+            tr = srt.Tracer(0)
+            tr.upload(phi.to("cuda:0"), fields.interleave_shift(shift).to("cuda:0"))
+            it = dict(tracer=tr, nside=4, pix_begin=0, count=192, n_steps=5 + i,
+                      out=torch.full((6 * 192,), float("nan"), dtype=torch.float32, device="cuda:0"))
+            ref = torch.empty_like(it["out"])
+            tr.trace_device(4, it["n_steps"], ref, precision=srt.PREC_MIXED)
+            items.append(it)
+            refs.append(ref)
+        plan = srt.Plan(items, precision=srt.PREC_MIXED)
+        return plan, [it["out"] for it in items], refs
+
+    plan, outs, refs = make()
+    gc.collect()
+    for _ in range(2):
+        for o in outs:
+            o.fill_(float("nan"))
+        plan.launch()
+        torch.cuda.synchronize()
+        for o, r in zip(outs, refs):
+            assert torch.equal(o, r)
+    plan.close()

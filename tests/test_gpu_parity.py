@@ -200,12 +200,13 @@ def test_empty_trace_and_errors(srt, tracer):
         tracer.trace_device(NSIDE, 4, o, precision=7)
     assert ei.value.status == 1
     assert float(o.abs().max()) == 0.0  # rejected before any work
-    # a full sky at the largest Nside is more wavefronts than one launch holds:
-    # refused (SRT_ERR_UNSUPPORTED) before the output is touched
-    nmax = 1 << 20
-    rc = srt.lib().srt_trace(tracer._ctx, nmax, 0, 12 * nmax * nmax, None, 1, None, srt.PREC_MIXED,
-                             srt.ORDER_LINEAR, ctypes.c_void_p(o.data_ptr()), None, None)
-    assert rc == 6
+    # an Nside above 2^20 and a count beyond the sky are refused (SRT_ERR_INVALID_ARGUMENT) before the
+    # output is touched (a valid count is bounded only by the caller's output buffer, which the library
+    # cannot see: traces of 2^32 rays and more are split into several launches)
+    for nside, count in (((1 << 20) + 1, 12), (NSIDE, 12 * NSIDE * NSIDE + 1)):
+        rc = srt.lib().srt_trace(tracer._ctx, nside, 0, count, None, 1, None, srt.PREC_MIXED,
+                                 srt.ORDER_LINEAR, ctypes.c_void_p(o.data_ptr()), None, None)
+        assert rc == 1
     assert float(o.abs().max()) == 0.0
 
 
@@ -237,3 +238,37 @@ def test_periodic_wrap_pow2_and_general_grids(srt, oracle, fields, n):
         assert e["dA_max_rel"] <= 1e-5
         assert e["shear_rms"] <= 1e-4
     tr.close()
+
+
+@pytest.mark.parametrize("layout", ["taps", "bicubic"])
+def test_trace_split_into_launches(srt, tracer, snap, layout, monkeypatch):
+    """A trace larger than the launch limit runs as several launches (one launch may not hold 2^32
+    lanes): a context created with SRT_MAX_LAUNCH_RAYS = 512 (a 3072-ray sky in 6 launches, a
+    1000-ray pixel list in 2, the last one partial) gives the same maps as one launch, in every
+    form and lane order."""
+    import torch
+    print(f"[gpu] split launches, {layout}", flush=True)
+    monkeypatch.setenv("SRT_MAX_LAUNCH_RAYS", "512")
+    small = srt.Tracer(0)
+    monkeypatch.delenv("SRT_MAX_LAUNCH_RAYS")
+    one = tracer
+    if layout == "bicubic":
+        one = srt.Tracer(0)
+        one.set_layout(srt.LAYOUT_BICUBIC, 16)
+        one.upload(snap[0], snap[2])
+        small.set_layout(srt.LAYOUT_BICUBIC, 16)
+    small.upload(snap[0], snap[2])
+    rng = np.random.default_rng(5)
+    pix = rng.choice(12 * NSIDE * NSIDE, 1000, replace=False).astype(np.int64)
+    forms = ("fp64", "mixed", "ref") if layout == "taps" else ("mixed",)
+    for prec in forms:
+        for order in (srt.ORDER_LINEAR, srt.ORDER_NESTED):
+            p = srt.PRECISIONS[prec]
+            for kw in (dict(), dict(batch=5), dict(pix=pix)):
+                a = one.trace(NSIDE, STEPS, precision=p, order=order, **kw)
+                b = small.trace(NSIDE, STEPS, precision=p, order=order, **kw)
+                assert np.array_equal(a, b), (prec, order, kw.keys())
+    small.close()
+    if one is not tracer:
+        one.close()
+    torch.cuda.synchronize()
