@@ -167,6 +167,23 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
 int srt_effective_order(int64_t nside, int64_t pix_begin, int64_t count, int has_pix_list, int ray_order,
                         int* effective);
 
+/* srt_trace without device allocations.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and maps as srt_trace.  The scratch of a sorted lane
+ * order (SRT_ORDER_NESTED other than over the full sky at a power-of-two
+ * Nside) lives in a fixed 48 MiB device array that the library holds on every
+ * GPU it runs on until the process ends; it covers at most 2^20 rays, and a
+ * larger sorted request fails with SRT_ERR_UNSUPPORTED (use srt_trace, or
+ * SRT_ORDER_LINEAR, which needs no scratch).  An SRT_PREC_FP64 trace needs
+ * the snapshot's FP64 grid copies, which an srt_trace with SRT_PREC_FP64
+ * builds; without them this call fails with SRT_ERR_UNSUPPORTED.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (any
+ * context, any stream) race. */
+int srt_trace_noalloc(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+                      int n_steps, const double* observer, int precision, int ray_order, float* out,
+                      double* final_state, void* stream);
+
 /* Synchronous host-memory variant of srt_trace for callers without device
  * buffers (the CLI): pix_list, out (6 * count floats) and final_state are host
  * pointers; pix_list and final_state may be null. */

@@ -45,7 +45,7 @@ EXPORTS = ("srt_create", "srt_destroy", "srt_strerror", "srt_last_error", "srt_u
            "srt_get_layout", "srt_trace",
            "srt_trace_host", "srt_trace_adjoint", "srt_plan_create", "srt_plan_destroy", "srt_plan_set_order", "srt_plan_configure",
            "srt_plan_get_config", "srt_plan_set_schedule", "srt_plan_get_schedule", "srt_plan_autotune", "srt_plan_autotune_brick", "srt_plan_launch", "srt_plan_last_ms", "srt_last_trace_ms",
-           "srt_grid_info", "srt_version", "srt_effective_order")
+           "srt_grid_info", "srt_version", "srt_effective_order", "srt_trace_noalloc")
 
 
 class BatchItem(ctypes.Structure):
@@ -95,6 +95,7 @@ def lib():
         L.srt_get_layout.argtypes = [P, P, P, P]
         L.srt_trace.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P, P]
         L.srt_trace_host.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P]
+        L.srt_trace_noalloc.argtypes = [P, i64, i64, i64, P, i32, P, i32, i32, P, P, P]
         L.srt_trace_adjoint.argtypes = [P, i64, i64, i64, P, i32, P, i32, P, P, P, P, P]
         L.srt_plan_create.argtypes = [P, i32, i32, ctypes.POINTER(P)]
         L.srt_plan_destroy.argtypes = [P]

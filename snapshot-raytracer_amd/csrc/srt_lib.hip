@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -803,6 +804,54 @@ void launch_trace(bool pow2, hipStream_t st, const TraceArgs& A_in, int64_t max_
 
 }  // namespace
 
+// ----------------------------------------------------- fixed scratch ----
+// Device scratch of srt_trace_noalloc, for callers that cannot allocate
+// device memory: the sorted lane order's lane -> slot map (4 B per ray) and
+// its sort space (keys, a second key buffer, slot indices and the radix
+// sort's temporary storage, about 20 B per ray) for partial-sky NESTED traces
+// of up to kNoAllocMaxSortedRays rays.  Every GPU the library runs on holds
+// one copy of the array until the process ends.  Host code never names the
+// array except to take its address: the address of each device's copy comes
+// from hipGetSymbolAddress (g_scratch, filled by srt_create), and only that
+// address is passed to kernels.  Calls that use it at the same time on one
+// GPU race; nothing in it needs to start at zero (the sort writes every key
+// and slot it reads).
+constexpr size_t kScratchBytes = size_t(48) << 20;  // 48 MiB
+constexpr int64_t kNoAllocMaxSortedRays = int64_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch[kMaxScratchDevices];  // per device: address of its copy of `scratch`, or null
+
+// Takes the address of the current device's copy of `scratch` (once per
+// device); `device` must be current.
+int setup_scratch(int device) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(SRT_ERR_UNSUPPORTED, "device ordinal beyond the scratch table (" +
+                                             std::to_string(kMaxScratchDevices) + " devices)");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return SRT_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(SRT_ERR_DEVICE, std::string("hipGetSymbolAddress of the scratch array: ") + hipGetErrorName(e));
+    }
+    g_scratch[device] = p;
+    return SRT_OK;
+}
+
+void* scratch_address(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch[device];
+}
+
+}  // namespace
+
 // ------------------------------------------------------------------ C ABI ----
 struct srt_context {
     int device = 0;
@@ -992,6 +1041,10 @@ int srt_create(int device, srt_context** ctx) {
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_create: device " + std::to_string(device) + " of " +
                                                   std::to_string(ndev));
     SRT_HIP(hipSetDevice(device));
+    {
+        const int rc = setup_scratch(device);
+        if (rc != SRT_OK) return rc;
+    }
     srt_context* c = new srt_context();
     c->device = device;
     if (const char* g = std::getenv("SRT_XCD_GROUPS")) {
@@ -1144,21 +1197,30 @@ int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_by
     return SRT_OK;
 }
 
-int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
-              int n_steps, const double* observer, int precision, int ray_order, float* out, double* final_state,
-              void* stream) {
-    if (!ctx || !out) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: null context or output");
-    if (!ctx->grids) return fail(SRT_ERR_NOT_UPLOADED, "srt_trace: no snapshot uploaded");
-    if (nside <= 0 || nside > (1 << 20)) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: nside out of range");
-    if (n_steps < 0) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: n_steps < 0");
+}  // extern "C"
+
+namespace {
+
+// srt_trace and srt_trace_noalloc.  With no_alloc the sorted lane order's
+// lane -> slot map and sort space come from the device array `scratch`
+// (the per-device address in g_scratch), and a request that would need an
+// allocation (a larger sorted order, the FP64 grid copies) is refused.
+int trace_entry(const char* who, bool no_alloc, srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
+                const int64_t* pix_list, int n_steps, const double* observer, int precision, int ray_order,
+                float* out, double* final_state, void* stream) {
+    const std::string w(who);
+    if (!ctx || !out) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "null context or output");
+    if (!ctx->grids) return fail(SRT_ERR_NOT_UPLOADED, w + ": " + "no snapshot uploaded");
+    if (nside <= 0 || nside > (1 << 20)) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "nside out of range");
+    if (n_steps < 0) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "n_steps < 0");
     const int64_t npix = 12 * nside * nside;
-    if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: count out of range");
+    if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "count out of range");
     if (!pix_list && (pix_begin < 0 || pix_begin + count > npix))
-        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: pixel range outside the sky");
+        return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "pixel range outside the sky");
     if (ray_order != SRT_ORDER_NESTED && ray_order != SRT_ORDER_LINEAR)
-        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown ray order");
+        return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "unknown ray order");
     if (precision != SRT_PREC_FP64 && precision != SRT_PREC_MIXED && precision != SRT_PREC_REF)
-        return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
+        return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "unknown precision");
     SRT_HIP(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     // NESTED over the full sky at power-of-two Nside: lanes are NESTED pixels;
@@ -1167,25 +1229,43 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     bool sorted = false;
     ray_order = effective_order(nside, pix_begin, count, pix_list != nullptr, ray_order, &sorted);
     uint32_t* lane_slot = nullptr;
+    void* order_buf = ctx->order_buf;
+    size_t order_bytes = ctx->order_bytes;
     if (sorted) {
         const size_t map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
         const size_t work = srt::order_work_bytes(nside, count);
-        if (!work) return fail(SRT_ERR_DEVICE, "srt_trace: lane-order work size");
-        if (map_bytes + work > ctx->order_bytes) {
+        if (!work) return fail(SRT_ERR_DEVICE, w + ": " + "lane-order work size");
+        if (no_alloc) {
+            if (count > kNoAllocMaxSortedRays || map_bytes + work > kScratchBytes)
+                return fail(SRT_ERR_UNSUPPORTED,
+                            w + ": a sorted lane order over " + std::to_string(count) + " rays needs " +
+                                std::to_string(map_bytes + work) + " bytes of scratch; the fixed scratch holds " +
+                                std::to_string(kScratchBytes) + " bytes (" + std::to_string(kNoAllocMaxSortedRays) +
+                                " rays): use srt_trace, or SRT_ORDER_LINEAR");
+            order_buf = scratch_address(ctx->device);
+            if (!order_buf) return fail(SRT_ERR_DEVICE, w + ": no address of the scratch array on this device");
+            order_bytes = kScratchBytes;
+        } else if (map_bytes + work > ctx->order_bytes) {
             SRT_HIP(hipDeviceSynchronize());  // an earlier launch may still read the old map
             if (ctx->order_buf) SRT_HIP(hipFree(ctx->order_buf));
             ctx->order_buf = nullptr;
             ctx->order_bytes = 0;
             SRT_HIP(ctx_malloc(ctx, &ctx->order_buf, map_bytes + work));
             ctx->order_bytes = map_bytes + work;
+            order_buf = ctx->order_buf;
+            order_bytes = ctx->order_bytes;
         }
-        lane_slot = static_cast<uint32_t*>(ctx->order_buf);
+        lane_slot = static_cast<uint32_t*>(order_buf);
     }
     TraceArgs A;
     A.G.phi = ctx->grids;
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
     for (int a = 0; a < 3; ++a) A.G.b[a] = ctx->grids + (size_t)(1 + a) * pvox;
     if (precision == SRT_PREC_FP64) {
+        if (no_alloc && !ctx->grids64)
+            return fail(SRT_ERR_UNSUPPORTED, w + ": the FP64 kernel reads FP64 copies of the grids, which this "
+                                                 "snapshot does not have yet (srt_trace with SRT_PREC_FP64 "
+                                                 "builds them)");
         const int rc = ensure_grids64(ctx, st);
         if (rc != SRT_OK) return rc;
     }
@@ -1210,7 +1290,7 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
     if (lane_slot) {  // the sort is part of the timed trace
         const size_t map_bytes = ((size_t)count * sizeof(uint32_t) + 255) / 256 * 256;
         SRT_HIP(srt::order_lanes(nside, pix_begin, count, pix_list, lane_slot,
-                                 static_cast<char*>(ctx->order_buf) + map_bytes, ctx->order_bytes - map_bytes, st));
+                                 static_cast<char*>(order_buf) + map_bytes, order_bytes - map_bytes, st));
     }
     if (count > 0) {
         const bool pow2 = (ctx->n & (ctx->n - 1)) == 0;
@@ -1228,13 +1308,31 @@ int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count,
                 launch_trace<srt::FormRef>(pow2, st, A, ctx->max_launch_rays);
                 break;
             default:
-                return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace: unknown precision");
+                return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "unknown precision");
         }
         SRT_HIP(hipGetLastError());
     }
     SRT_HIP(hipEventRecord(ctx->ev_stop, st));
     ctx->timed = true;
     return SRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int srt_trace(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+              int n_steps, const double* observer, int precision, int ray_order, float* out, double* final_state,
+              void* stream) {
+    return trace_entry("srt_trace", false, ctx, nside, pix_begin, count, pix_list, n_steps, observer, precision,
+                       ray_order, out, final_state, stream);
+}
+
+int srt_trace_noalloc(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
+                      int n_steps, const double* observer, int precision, int ray_order, float* out,
+                      double* final_state, void* stream) {
+    return trace_entry("srt_trace_noalloc", true, ctx, nside, pix_begin, count, pix_list, n_steps, observer,
+                       precision, ray_order, out, final_state, stream);
 }
 
 int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
