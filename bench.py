@@ -34,16 +34,47 @@ FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12793.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
 HBM_BYTES_PER_LAUNCH = {"mixed": 28.44e9, "mixed_taps": 108.4e9, "fp64": 7.87e9}
-# The reference algorithm's count (oracle with an operation-counting scalar).
-REF_ALGO_FLOPS_PER_RAY_STEP = 29294.0
+# Algorithmic flops per ray-step, the roofline's work constant: the operation
+# count of a work-efficient form of the reference's mathematics (per axis the
+# Catmull-Rom weight vectors once, weight sets shared between the four fields of
+# a right-hand side, the 4 x 4 x 4 taps contracted x -> y -> z; the rest of the
+# right-hand side and RK4 as the reference has them), counted by running the FP64
+# oracle with an operation-counting scalar (oracle/opcount.cpp, variant 2) on
+# Nside 16, RING pixel 1000, 32 steps of the seed-1 64^3 snapshot.
+ALGO_FLOPS_PER_RAY_STEP = 12290.5
+# The same count for the reference's own form (three expanded Catmull-Rom
+# polynomials per row and pass, .cu:195-312), for comparison.
+REF_FORM_FLOPS_PER_RAY_STEP = 29298.5
 # MI355X published vector peaks (FP32 counts packed FP32; this build disables
-# packed FP32, so 78.6 TFLOP/s is the attainable FP32 ceiling).
+# packed FP32, so 78.6 TFLOP/s is the FP32 rate of the instructions it issues).
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3, "mixed_taps": 157.3, "ref": 157.3}
 # kernel name -> (precision, device layout it reads); None = the padded cubes
 # every context holds
 KERNELS = {"mixed": ("mixed", "bicubic"), "mixed_taps": ("mixed", "taps"), "fp64": ("fp64", None),
            "ref": ("ref", None)}
 UNPACKED_FP32_TFLOPS = 78.6
+
+
+def host_cpu():
+    """(threads the CPU baseline uses, CPU model): the process's CPU share (OMP_NUM_THREADS when set,
+    else the CPUs this process may run on) and the model name from /proc/cpuinfo."""
+    threads = 0
+    try:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        pass
+    if threads <= 0:
+        threads = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, model
 
 
 def parse():
@@ -63,7 +94,9 @@ def parse():
     p.add_argument("--order", default="nested", choices=["nested", "linear"])
     p.add_argument("--subset-frac", type=float, default=0.01)
     p.add_argument("--no-oracle", action="store_true", help="skip the CPU oracle leg (accuracy + cpu baseline)")
-    p.add_argument("--cpu-threads", type=int, default=16)
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU this process "
+                        "may run on)")
     p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch", "layouts"],
                    help="headline: 512^3 / Nside 1024 / 256 steps (the JSON line the driver reads); "
                         "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
@@ -510,43 +543,72 @@ def main():
     if not args.no_oracle:
         from oracle import oracle
         Fh = oracle.Fields(phi.cpu().numpy(), shift.cpu().numpy())
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
-        print(f"[bench] oracle fp64 on {subset.size} rays x {n_steps} steps, {threads} threads",
-              file=sys.stderr, flush=True)
+        threads, cpu_model = host_cpu()
+        if args.cpu_threads > 0:
+            threads = args.cpu_threads
+        visible = len(os.sched_getaffinity(0))
+        print(f"[bench] oracle fp64 and ref forms on {subset.size} rays x {n_steps} steps, {threads} threads "
+              f"({visible} CPUs visible, {cpu_model})", file=sys.stderr, flush=True)
         ref, sec = oracle.trace(Fh, nside, subset, n_steps, form=oracle.FORM_FP64, nthreads=threads)
-        # a further disjoint timed sample brings the CPU work to ~10-20 s
-        extra = np.setdiff1d(np.sort(np.random.default_rng(10).choice(rays, 2 * subset.size, replace=False)),
-                             subset)[: int(1.5 * subset.size)]
-        _, sec2 = oracle.trace(Fh, nside, extra, n_steps, form=oracle.FORM_FP64, nthreads=threads)
-        nr = subset.size + extra.size
-        cpu_baseline = {"value": nr * n_steps / (sec + sec2), "unit": "ray-steps/s", "cores": threads,
-                        "kind": "port",
-                        "sample": f"FP64 host oracle (OpenMP C++, {threads} threads) on {nr} seeded pixels "
-                                  f"(the 1% accuracy subset plus {extra.size} more) x {n_steps} steps of the same "
-                                  f"snapshot, {sec + sec2:.1f} s"}
+        _, sec_ref = oracle.trace(Fh, nside, subset, n_steps, form=oracle.FORM_REF, nthreads=threads)
+        # the 64^3 / Nside 16 / 32-step snapshot (seed 1), full sky, both forms
+        phi1, shift1 = fields.snapshot(64, 1)
+        F1 = oracle.Fields(phi1.numpy(), shift1.numpy())
+        pix1 = np.arange(12 * 16 * 16)
+        small = {}
+        for form_name, form in (("fp64", oracle.FORM_FP64), ("ref", oracle.FORM_REF)):
+            _, s1 = oracle.trace(F1, 16, pix1, 32, form=form, nthreads=threads)
+            small[form_name] = pix1.size * 32 / s1
+        rs = subset.size * n_steps
+        cpu_baseline = {"value": rs / sec_ref, "unit": "ray-steps/s", "cores": threads, "kind": "port",
+                        "sample": f"host oracle in the reference's arithmetic (REF form: float interpolation, "
+                                  f"double state; OpenMP C++, {threads} threads) on the seeded 1% pixel subset "
+                                  f"({subset.size} rays) x {n_steps} steps of the same snapshot, {sec_ref:.1f} s",
+                        "fp64_form": {"value": rs / sec, "seconds": sec},
+                        "cpu_model": cpu_model, "cpus_visible": visible,
+                        "snapshot_64cube_nside16_32steps_full_sky": small}
         for name, m in maps_sub.items():
             accuracy[name] = map_errors(m, ref)
 
     head = "mixed" if "mixed" in results else next(iter(results))
     r = results[head]
-    flops = FLOPS_PER_RAY_STEP.get(head)
     roofline = None
-    if flops:
-        achieved = flops * rays * n_steps / (r["kernel_ms_avg"] / 1000.0) / 1e12
-        roofline = {"bound": "mfma", "compute_unit": "VALU vector FP32/FP64 (nothing on this path is a "
-                                                      "matrix product)",
+    if head in PEAK_TFLOPS:
+        rs_launch = rays * n_steps
+        # the mixed kernel reads bicubic records built once per snapshot: one build is charged to the trace
+        build_ms = 1000.0 * layout_s.get("bicubic", 0.0) if KERNELS[head][1] == "bicubic" else 0.0
+        charged_ms = r["kernel_ms_avg"] + build_ms
+        achieved = ALGO_FLOPS_PER_RAY_STEP * rs_launch / (charged_ms / 1000.0) / 1e12
+        roofline = {"bound": "valu",
+                    "compute_unit": "vector ALU (FP32 interpolation, FP64 state); nothing on this path is a "
+                                    "matrix product, so MFMA does not apply",
                     "achieved": achieved, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
                     "frac": achieved / PEAK_TFLOPS[head],
-                    "frac_of_unpacked_fp32_peak": achieved / UNPACKED_FP32_TFLOPS,
                     "traffic": HBM_BYTES_PER_LAUNCH.get(head),
                     "hbm_frac": (HBM_BYTES_PER_LAUNCH[head] / (r["kernel_ms_avg"] / 1000.0) / 8e12
                                  if head in HBM_BYTES_PER_LAUNCH else None),
-                    "flops_per_ray_step": flops, "reference_algorithm_flops_per_ray_step":
-                        REF_ALGO_FLOPS_PER_RAY_STEP,
+                    "algorithmic_flops_per_ray_step": ALGO_FLOPS_PER_RAY_STEP,
+                    "reference_form_flops_per_ray_step": REF_FORM_FLOPS_PER_RAY_STEP,
+                    "ray_steps_per_launch": rs_launch,
+                    "kernel_ms": r["kernel_ms_avg"], "record_build_ms": build_ms, "charged_ms": charged_ms,
+                    "frac_kernel_time_only": ALGO_FLOPS_PER_RAY_STEP * rs_launch / (r["kernel_ms_avg"] / 1e3)
+                                             / 1e12 / PEAK_TFLOPS[head],
                     "kernel": "trace_kernel<mixed, bicubic records>" if head == "mixed" else f"trace_kernel<{head}>"}
+        if head in FLOPS_PER_RAY_STEP:
+            ex = FLOPS_PER_RAY_STEP[head] * rs_launch / (r["kernel_ms_avg"] / 1000.0) / 1e12
+            roofline["executed"] = {"flops_per_ray_step": FLOPS_PER_RAY_STEP[head], "tflops": ex,
+                                    "frac_of_peak": ex / PEAK_TFLOPS[head],
+                                    "frac_of_unpacked_fp32_rate": ex / UNPACKED_FP32_TFLOPS}
+        if "fp64" in results and head != "fp64":
+            a64 = ALGO_FLOPS_PER_RAY_STEP * rs_launch / (results["fp64"]["kernel_ms_avg"] / 1000.0) / 1e12
+            roofline["fp64_kernel"] = {"achieved": a64, "peak": PEAK_TFLOPS["fp64"], "frac": a64 / PEAK_TFLOPS["fp64"]}
     line = {
         "metric": "ray-steps/s at 512^3, Nside 1024, 256 steps (mixed-precision kernel)",
         "value": r["ray_steps_per_s"],
+        "value_excludes": "upload of the snapshot and its per-snapshot bicubic record build (see "
+                          "value_with_record_build), like the metric's definition",
+        "value_with_record_build": (rays * n_steps / ((r["ms_per_step"] + 1000.0 * layout_s.get("bicubic", 0.0))
+                                                      / 1000.0) if KERNELS[head][1] == "bicubic" else None),
         "unit": "ray-steps/s",
         "n_gpus": 1,
         "steps": args.steps,

@@ -51,7 +51,8 @@ def bind(path: str):
     L.orc_f16_check_range.restype = ctypes.c_uint64
     L.orc_max_threads.restype = ctypes.c_int
     if hasattr(L, "orc_count_ops"):
-        L.orc_count_ops.argtypes = [P, P, P, P, i64, i64, i64, i64, i64, ctypes.c_int, P, P]
+        L.orc_count_ops.argtypes = [P, P, P, P, i64, i64, i64, i64, i64, ctypes.c_int, P, P, ctypes.c_int]
+        L.orc_tricubic_weights.argtypes = [P, i64, i64, i64, P, i64, P]
     return L
 
 
@@ -153,16 +154,32 @@ def trace(fields: Fields, nside: int, pix, n_steps: int, observer=None, form=FOR
     return out, float(sec[0])
 
 
-def count_ops(fields: Fields, nside: int, pix: int, n_steps: int, observer=None) -> dict:
-    """Floating-point operations per ray-step of the FP64 oracle algorithm."""
+def count_ops(fields: Fields, nside: int, pix: int, n_steps: int, observer=None, variant: int = 0) -> dict:
+    """Floating-point operations per ray-step of the FP64 oracle algorithm.
+
+    variant 0: the reference's interpolation form (three expanded Catmull-Rom
+    polynomials per row and pass); 1: the work-efficient weights form of the
+    same interpolant (opcount.cpp, WeightsTricubic); 2: the weights form with
+    the weight sets shared between the four fields of a right-hand side."""
     if observer is None:
         observer = (fields.nx * 0.5, fields.ny * 0.5, fields.nz * 0.5)
     obs = np.ascontiguousarray(observer, dtype=np.float64)
     c = np.zeros(4, np.float64)
     p = fields.ptrs()
-    lib().orc_count_ops(p[0], p[1], p[2], p[3], fields.nx, fields.ny, fields.nz, nside, pix, n_steps,
-                        _p(obs), _p(c))
+    rc = lib().orc_count_ops(p[0], p[1], p[2], p[3], fields.nx, fields.ny, fields.nz, nside, pix, n_steps,
+                             _p(obs), _p(c), variant)
+    if rc:
+        raise ValueError("invalid count_ops arguments")
     return {"add": c[0], "mul": c[1], "div": c[2], "sqrt": c[3], "total": float(c.sum())}
+
+
+def tricubic_weights(grid, pos) -> np.ndarray:
+    """The weights-form interpolation (FP64) at positions pos (n x 3); n x 10 like tricubic()."""
+    nz, ny, nx = grid.shape
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    out = np.empty((pos.shape[0], 10), np.float64)
+    lib().orc_tricubic_weights(_p(_f32(grid)), nx, ny, nz, _p(pos), pos.shape[0], _p(out))
+    return out
 
 
 def f32_to_f16(a) -> np.ndarray:

@@ -261,7 +261,17 @@ inline int hidx(int i, int j) {
     return k[i][j];
 }
 
-template <class S, class I>
+// The interpolation the right-hand side calls: the restatement above.  (The
+// operation count in opcount.cpp substitutes a weights-form evaluation of the
+// same interpolant to count a work-efficient algorithm.)
+struct OracleTricubic {
+    template <class I, class S>
+    static Interp<I> eval(const Grid& G, S x, S y, S z) {
+        return tricubic<I, S>(G, x, y, z);
+    }
+};
+
+template <class S, class I, class Tri = OracleTricubic>
 inline void rhs(const Fields& F, const S* y, S* dy) {
     using std::sqrt;
     S n[3], e1[3], e2[3];
@@ -291,7 +301,7 @@ inline void rhs(const Fields& F, const S* y, S* dy) {
     for (int a = 0; a < 3; ++a) {
         S p[3] = {y[0], y[1], y[2]};
         p[a] = p[a] - S(0.5);
-        const Interp<I> r = tricubic<I, S>(F.b[a], p[0], p[1], p[2]);
+        const Interp<I> r = Tri::template eval<I, S>(F.b[a], p[0], p[1], p[2]);
         Bv[a] = r.v;
         S g[3] = {S(r.g[0]), S(r.g[1]), S(r.g[2])};
         S H[3][3];
@@ -320,7 +330,7 @@ inline void rhs(const Fields& F, const S* y, S* dy) {
     }
 
     // potential at x (.cu:462)
-    const Interp<I> f = tricubic<I, S>(F.phi, y[0], y[1], y[2]);
+    const Interp<I> f = Tri::template eval<I, S>(F.phi, y[0], y[1], y[2]);
     const S phi = S(f.v);
     const S g[3] = {S(f.g[0]), S(f.g[1]), S(f.g[2])};
     S H[3][3];
@@ -379,23 +389,23 @@ inline void rhs(const Fields& F, const S* y, S* dy) {
 // ------------------------------------------------------------------ RK4 ----
 // k1 = f(y), k2 = f(y + k1/2), k3 = f(y + k2/2), k4 = f(y + k3),
 // y += ((k1 + 2 k2 + 2 k3) + k4) / 6, accumulated in that order (.cu:513-550).
-template <class S, class I>
+template <class S, class I, class Tri = OracleTricubic>
 inline void rk4_integrate(const Fields& F, S* y, int n_steps) {
     S tmp[19], k[19], acc[19];
     for (int step = 0; step < n_steps; ++step) {
-        rhs<S, I>(F, y, acc);
+        rhs<S, I, Tri>(F, y, acc);
         for (int j = 0; j < 19; ++j) tmp[j] = y[j] + S(0.5) * acc[j];
-        rhs<S, I>(F, tmp, k);
+        rhs<S, I, Tri>(F, tmp, k);
         for (int j = 0; j < 19; ++j) {
             tmp[j] = y[j] + S(0.5) * k[j];
             acc[j] = acc[j] + S(2) * k[j];
         }
-        rhs<S, I>(F, tmp, k);
+        rhs<S, I, Tri>(F, tmp, k);
         for (int j = 0; j < 19; ++j) {
             tmp[j] = y[j] + k[j];
             acc[j] = acc[j] + S(2) * k[j];
         }
-        rhs<S, I>(F, tmp, k);
+        rhs<S, I, Tri>(F, tmp, k);
         for (int j = 0; j < 19; ++j) y[j] = y[j] + (acc[j] + k[j]) / S(6);
     }
 }

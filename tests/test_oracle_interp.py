@@ -183,3 +183,31 @@ def test_ref_form_offsets(oracle, rgrid):
     assert (np.abs(ref - at_f32) / scale).max() < 2e-6
     # ... which differs from the exact FP64 position measurably
     assert (np.abs(ref - exact) / scale).max() > 1e-7
+
+
+def test_weights_form_equals_restatement(oracle, fields):
+    """The work-efficient weights form (weight vectors per axis, one x -> y -> z contraction) whose
+    operations give the roofline's algorithmic flop count evaluates the same interpolant as the
+    restatement of .cu:215-312, including across the periodic wrap."""
+    phi, _ = fields.snapshot(32, 21)
+    g = phi.numpy()
+    pos = np.random.default_rng(22).uniform(-40, 75, (3000, 3))
+    pos[:10] = np.floor(pos[:10])  # exact nodes
+    a = oracle.tricubic(g, pos)
+    b = oracle.tricubic_weights(g, pos)
+    scale = np.abs(a).max(axis=0)
+    assert (np.abs(a - b) / scale).max() < 1e-13
+
+
+def test_algorithmic_flop_constant(oracle, fields):
+    """The roofline's algorithmic flops per ray-step (bench.ALGO_FLOPS_PER_RAY_STEP) is the operation
+    count of the work-efficient form with shared weight sets; the reference's expanded-polynomial form
+    counts about 2.4x as many."""
+    import bench
+    phi, shift = fields.snapshot(64, 1)
+    F = oracle.Fields(phi.numpy(), shift.numpy())
+    counts = [oracle.count_ops(F, 16, 1000, 32, variant=v)["total"] for v in (0, 1, 2)]
+    print("flops per ray-step (reference form, weights form, shared weight sets):", counts)
+    assert counts[2] == bench.ALGO_FLOPS_PER_RAY_STEP
+    assert counts[0] == bench.REF_FORM_FLOPS_PER_RAY_STEP
+    assert counts[2] < counts[1] < counts[0]
