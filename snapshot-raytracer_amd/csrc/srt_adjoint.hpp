@@ -1,14 +1,21 @@
 // Reverse-mode (adjoint) building blocks of the FP64 ray integrator.
 //
-// For a loss L(maps) the adjoint kernel propagates dL/dy backwards through the
-// RK4 steps (.cu:513-550), the right-hand side (.cu:315-510) and the tricubic
-// interpolation (.cu:215-312), and scatters dL/d(grid value) with atomics.
-// Everything here is FP64.  The right-hand side is restated with its
-// intermediates kept (rhs_forward) so its vector-Jacobian product (rhs_vjp)
-// can be written out step by step; the interpolation adjoint has two parts:
-// the transpose of the tap contraction (gradient with respect to the 64 tap
-// values) and the derivative with respect to the position (third derivatives
-// of the Catmull-Rom interpolant contracted with the Hessian adjoint).
+// For a loss L(maps) the adjoint propagates dL/dy backwards through the RK4
+// steps (.cu:513-550), the right-hand side (.cu:315-510) and the tricubic
+// interpolation (.cu:215-312), and scatters dL/d(grid value) into the grids.
+// Everything here is FP64.  The work is split so that each part runs with the
+// parallelism it has:
+//   * the interpolations of a ray's four fields (potential, B_x, B_y, B_z) are
+//     independent: field_stencil / field_interp evaluate one field, so four
+//     lanes can serve one ray (value, gradient, Hessian and, for the reverse
+//     sweep, the third derivatives);
+//   * the right-hand side is restated with its intermediates kept
+//     (rhs_forward_from) from the four fields' interpolants, and its
+//     vector-Jacobian product (rhs_vjp_bars) is pure algebra: it takes the
+//     stored third derivatives for the position adjoint and returns, per field,
+//     the adjoints of the interpolated value, gradient and Hessian;
+//   * the grid gradient of those adjoints is the transpose of the 64-tap
+//     contraction (scatter_taps), independent for every (ray, stage, field).
 #pragma once
 
 #include "srt_device.hpp"
@@ -42,52 +49,14 @@ __device__ __forceinline__ Axis4 cr_axis4(double t) {
     return r;
 }
 
-// Interpolated value, gradient, Hessian (xx xy xz yy yz zz) and third
-// derivatives (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz).
-struct Interp3 {
-    double v, g[3], h[6], t[10];
-};
-
-// derivative orders (x, y, z) of the 20 outputs: value, gradient, Hessian, third derivatives
-__constant__ static const signed char kOrders[20][3] = {
+// derivative orders (x, y, z) of the 20 interpolation outputs: value, gradient,
+// Hessian (xx xy xz yy yz zz), third derivatives (xxx xxy xxz xyy xyz xzz yyy
+// yyz yzz zzz)
+constexpr signed char kOrders[20][3] = {
     {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2},
     {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
 
-__device__ __forceinline__ Interp3 tricubic3(const float* __restrict__ base, size_t py, size_t pz, const Axis4& ax,
-                                             const Axis4& ay, const Axis4& az) {
-    double o[20];
-    for (int k = 0; k < 20; ++k) o[k] = 0.0;
-    // the z slices are a loop (not unrolled): the code of the reverse sweep
-    // stays a few thousand instructions, with fewer values live at once
-#pragma unroll 1
-    for (int c = 0; c < 4; ++c) {
-        // y-pass results for this z slice, indexed [ox][oy]
-        double Y[4][4];
-        for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) Y[i][j] = 0.0;
-        for (int b = 0; b < 4; ++b) {
-            const gptr<const float> row = (gptr<const float>)(base + b * py + c * pz);
-            const double p[4] = {row[0], row[1], row[2], row[3]};
-            double X[4];
-            for (int ox = 0; ox < 4; ++ox)
-                X[ox] = ax.w[ox][0] * p[0] + ax.w[ox][1] * p[1] + ax.w[ox][2] * p[2] + ax.w[ox][3] * p[3];
-            for (int ox = 0; ox < 4; ++ox)
-                for (int oy = 0; ox + oy < 4; ++oy) Y[ox][oy] += ay.w[oy][b] * X[ox];
-        }
-        for (int k = 0; k < 20; ++k) {
-            const int ox = kOrders[k][0], oy = kOrders[k][1], oz = kOrders[k][2];
-            o[k] += az.w[oz][c] * Y[ox][oy];
-        }
-    }
-    Interp3 r;
-    r.v = o[0];
-    for (int i = 0; i < 3; ++i) r.g[i] = o[1 + i];
-    for (int i = 0; i < 6; ++i) r.h[i] = o[4 + i];
-    for (int i = 0; i < 10; ++i) r.t[i] = o[10 + i];
-    return r;
-}
-
-// index of the third derivative d3/(dx_i dx_j dx_k) in Interp3::t
+// index of the third derivative d3/(dx_i dx_j dx_k) among the ten third derivatives (kOrders 10..19)
 __device__ __forceinline__ int t3idx(int i, int j, int k) {
     // sort i <= j <= k
     if (i > j) { const int s = i; i = j; j = s; }
@@ -115,19 +84,26 @@ constexpr int kGradTileCells = 1024;
 __shared__ double grad_tile[kGradTileCells];
 __shared__ int grad_box[6];  // min x, y, z; max x, y, z of the stencil origins
 
+// Calls f(a, b, c, dL/d tap) for the 64 taps (a, b, c) of a stencil whose
+// interpolation outputs have adjoints bar[10]; with c_lo <= c_hi only the z
+// slices c in [c_lo, c_hi] (others are skipped).
 template <class F>
 __device__ __forceinline__ void for_each_tap(const Axis4& ax, const Axis4& ay, const Axis4& az, const double bar[10],
-                                             F&& f) {
+                                             F&& f, int c_lo = 0, int c_hi = 3) {
     // z-pass transpose: per slice c, adjoints of the six y-pass sums; the slices
-    // are a loop (not unrolled), which keeps the reverse sweep's code small
+    // are a loop (not unrolled), which keeps the code small
 #pragma unroll 1
-    for (int c = 0; c < 4; ++c) {
-        const double yv = az.w[0][c] * bar[0] + az.w[1][c] * bar[3] + az.w[2][c] * bar[9];
-        const double yd = az.w[0][c] * bar[2] + az.w[1][c] * bar[8];
-        const double ydd = az.w[0][c] * bar[7];
-        const double xd = az.w[0][c] * bar[1] + az.w[1][c] * bar[6];
-        const double xdyd = az.w[0][c] * bar[5];
-        const double xdd = az.w[0][c] * bar[4];
+    for (int c = max(c_lo, 0); c <= min(c_hi, 3); ++c) {
+        // the slice's z weights by constant indices (a runtime index would put az in scratch)
+        const double w0 = az.w[0][0] * (c == 0) + az.w[0][1] * (c == 1) + az.w[0][2] * (c == 2) + az.w[0][3] * (c == 3);
+        const double w1 = az.w[1][0] * (c == 0) + az.w[1][1] * (c == 1) + az.w[1][2] * (c == 2) + az.w[1][3] * (c == 3);
+        const double w2 = az.w[2][0] * (c == 0) + az.w[2][1] * (c == 1) + az.w[2][2] * (c == 2) + az.w[2][3] * (c == 3);
+        const double yv = w0 * bar[0] + w1 * bar[3] + w2 * bar[9];
+        const double yd = w0 * bar[2] + w1 * bar[8];
+        const double ydd = w0 * bar[7];
+        const double xd = w0 * bar[1] + w1 * bar[6];
+        const double xdyd = w0 * bar[5];
+        const double xdd = w0 * bar[4];
         for (int b = 0; b < 4; ++b) {
             const double rv = ay.w[0][b] * yv + ay.w[1][b] * yd + ay.w[2][b] * ydd;
             const double rd = ay.w[0][b] * xd + ay.w[1][b] * xdyd;
@@ -137,48 +113,113 @@ __device__ __forceinline__ void for_each_tap(const Axis4& ax, const Axis4& ay, c
     }
 }
 
-__device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, int cx, int cy, int cz, size_t py,
-                                             size_t pz, const Axis4& ax, const Axis4& ay, const Axis4& az,
-                                             const double bar[10]) {
+// Box of the wavefront's stencils: origin (tap (-1,-1,-1)) and extent in
+// cells, reduced over the active lanes from each lane's range of stencil
+// origins [lo, hi].  The same in every lane.
+struct TileBox {
+    int x0, y0, z0, nx, ny, nz;
+    __device__ int cells() const { return nx * ny * nz; }
+};
+
+__device__ __forceinline__ TileBox wave_box(const int lo[3], const int hi[3]) {
     const int lane = threadIdx.x & 63;
     const uint64_t active = __ballot(1);
-    const int n_active = __popcll(active);
-    const int rank = __popcll(active & ((uint64_t(1) << lane) - 1));
-    if (rank == 0) {
+    if (__popcll(active & ((uint64_t(1) << lane) - 1)) == 0) {
         grad_box[0] = grad_box[1] = grad_box[2] = 0x7fffffff;
         grad_box[3] = grad_box[4] = grad_box[5] = -1;
     }
     __syncthreads();
-    (void)__hip_atomic_fetch_min(&grad_box[0], cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    (void)__hip_atomic_fetch_min(&grad_box[1], cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    (void)__hip_atomic_fetch_min(&grad_box[2], cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    (void)__hip_atomic_fetch_max(&grad_box[3], cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    (void)__hip_atomic_fetch_max(&grad_box[4], cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    (void)__hip_atomic_fetch_max(&grad_box[5], cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int k = 0; k < 3; ++k) {
+        (void)__hip_atomic_fetch_min(&grad_box[k], lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        (void)__hip_atomic_fetch_max(&grad_box[3 + k], hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     __syncthreads();
-    const int x0 = grad_box[0], y0 = grad_box[1], z0 = grad_box[2];
-    const int nx = grad_box[3] - x0 + 4, ny = grad_box[4] - y0 + 4, nz = grad_box[5] - z0 + 4;
+    TileBox B;
+    B.x0 = grad_box[0];
+    B.y0 = grad_box[1];
+    B.z0 = grad_box[2];
+    B.nx = grad_box[3] - B.x0 + 4;
+    B.ny = grad_box[4] - B.y0 + 4;
+    B.nz = grad_box[5] - B.z0 + 4;
     __syncthreads();  // the box is read before the next call resets it
-    const int cells = nx * ny * nz;  // the same in every lane
-    if (cells <= kGradTileCells) {
-        for (int q = rank; q < cells; q += n_active) grad_tile[q] = 0.0;
-        __syncthreads();
-        const int t0 = (cx - x0) + nx * ((cy - y0) + ny * (cz - z0));
-        for_each_tap(ax, ay, az, bar, [&](int a, int b, int c, double pb) {
-            (void)__hip_atomic_fetch_add(&grad_tile[t0 + a + nx * (b + ny * c)], pb, __ATOMIC_RELAXED,
+    return B;
+}
+
+// Clears the first `cells` entries of the LDS tile (the active lanes share the work).
+__device__ __forceinline__ void tile_clear(int cells) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t active = __ballot(1);
+    const int n_active = __popcll(active), rank = __popcll(active & ((uint64_t(1) << lane) - 1));
+    for (int q = rank; q < cells; q += n_active) grad_tile[q] = 0.0;
+    __syncthreads();
+}
+
+// Adds the taps of one stencil (origin cx, cy, cz) that lie inside box B
+// (its z range; x and y are whole) into the tile of B.
+__device__ __forceinline__ void tile_add(const TileBox& B, int cx, int cy, int cz, const Axis4& ax, const Axis4& ay,
+                                         const Axis4& az, const double bar[10]) {
+    const int t0 = (cx - B.x0) + B.nx * ((cy - B.y0) + B.ny * (cz - B.z0));
+    for_each_tap(
+        ax, ay, az, bar,
+        [&](int a, int b, int c, double pb) {
+            (void)__hip_atomic_fetch_add(&grad_tile[t0 + a + B.nx * (b + B.ny * c)], pb, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-        });
-        __syncthreads();
-        for (int q = rank; q < cells; q += n_active) {
-            const double v = grad_tile[q];
-            if (v != 0.0) {
-                const int qx = q % nx, qy = (q / nx) % ny, qz = q / (nx * ny);
-                const gptr<double> dst = (gptr<double>)(gbase_dev + (size_t)(x0 + qx) + (size_t)(y0 + qy) * py +
-                                                        (size_t)(z0 + qz) * pz);
-                (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, v);
-            }
+        },
+        B.z0 - cz, B.z0 + B.nz - 1 - cz);
+}
+
+// Adds every non-zero tile entry to the padded gradient grid (one global FP64
+// atomic per touched cell) after the tile is complete.
+__device__ __forceinline__ void tile_flush(double* __restrict__ gbase_dev, const TileBox& B, size_t py, size_t pz) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t active = __ballot(1);
+    const int n_active = __popcll(active), rank = __popcll(active & ((uint64_t(1) << lane) - 1));
+    __syncthreads();
+    const int cells = B.cells(), nxy = B.nx * B.ny;
+    for (int q = rank; q < cells; q += n_active) {
+        const double v = grad_tile[q];
+        if (v != 0.0) {
+            const int qz = q / nxy, r = q - qz * nxy, qy = r / B.nx, qx = r - qy * B.nx;
+            const gptr<double> dst = (gptr<double>)(gbase_dev + (size_t)(B.x0 + qx) + (size_t)(B.y0 + qy) * py +
+                                                    (size_t)(B.z0 + qz) * pz);
+            (void)__builtin_amdgcn_global_atomic_fadd_f64(dst, v);
         }
-        __syncthreads();  // the tile is read before the next call clears it
+    }
+    __syncthreads();  // the tile is read before the next call clears it
+}
+
+// z slabs of box B that fit the tile: slab k covers z planes
+// [z0 + k dz, z0 + min((k + 1) dz, nz)); 0 when one z plane is larger than the
+// tile (a box across the periodic wrap), and then the caller falls back to
+// global atomics.
+__device__ __forceinline__ int slab_planes(const TileBox& B) {
+    const int nxy = B.nx * B.ny;
+    return nxy > kGradTileCells ? 0 : kGradTileCells / nxy;
+}
+
+__device__ __forceinline__ TileBox slab(const TileBox& B, int zlo, int dz) {
+    TileBox P = B;
+    P.z0 = B.z0 + zlo;
+    P.nz = min(dz, B.nz - zlo);
+    return P;
+}
+
+// One stencil per lane: the tap gradients through the tile, in as many z
+// slabs of the wavefront's box as the tile needs, else (a box wider than the
+// tile in one plane) one global atomic per lane and tap.
+__device__ __forceinline__ void scatter_taps(double* __restrict__ gbase_dev, int cx, int cy, int cz, size_t py,
+                                             size_t pz, const Axis4& ax, const Axis4& ay, const Axis4& az,
+                                             const double bar[10]) {
+    const int c[3] = {cx, cy, cz};
+    const TileBox B = wave_box(c, c);
+    const int dz = slab_planes(B);
+    if (dz > 0) {
+        for (int zlo = 0; zlo < B.nz; zlo += dz) {
+            const TileBox P = slab(B, zlo, dz);
+            tile_clear(P.cells());
+            tile_add(P, cx, cy, cz, ax, ay, az, bar);
+            tile_flush(gbase_dev, P, py, pz);
+        }
     } else {
         const size_t off = (size_t)cx + (size_t)cy * py + (size_t)cz * pz;
         for_each_tap(ax, ay, az, bar, [&](int a, int b, int c, double pb) {
@@ -201,71 +242,89 @@ struct RhsTape {
     bool br;
 };
 
-// One field's stencil: offset of tap (-1,-1,-1) in the padded grid and the
-// cell offsets along x, y, z.  The Catmull-Rom weights are recomputed where
-// they are used, so that four stencils do not hold 4 x 48 weights live.
+// One field's stencil: offset of tap (-1,-1,-1) in the padded grid, its
+// padded index along x, y, z, and the cell offsets along x, y, z.
 struct Stencil {
     size_t off;
-    int c[3];  // padded x, y, z index of tap (-1, -1, -1)
+    int c[3];
     double t[3];
 };
 
-__device__ __forceinline__ void locate_stencils(const Grids& G, const double x[3], Stencil st[4]) {
-    int i0[3], i1[3];
-    double t0[3], t1[3];
+// Stencil of field f at position x: f = 0 is the potential, sampled at x;
+// f = 1 + a is shift component a, sampled at x - e_a / 2 (.cu:377, 406, 432).
+// The staggered cell along a is the unstaggered one or the one below it, and
+// its offset t -/+ 1/2 is exact.
+__device__ __forceinline__ Stencil field_stencil(const Grids& G, const double x[3], int f) {
+    Stencil st;
     for (int a = 0; a < 3; ++a) {
         double t;
-        i0[a] = locate(x[a], G.n, G.inv_n, t);
-        t0[a] = t;
-        const bool upper = t >= 0.5;
-        const int i = upper ? i0[a] : i0[a] - 1;
-        i1[a] = i < 0 ? i + G.n : i;
-        t1[a] = upper ? t - 0.5 : t + 0.5;
+        const int i0 = locate(x[a], G.n, G.inv_n, t);
+        if (f == a + 1) {
+            const bool upper = t >= 0.5;
+            const int i = upper ? i0 : i0 - 1;
+            st.c[a] = i < 0 ? i + G.n : i;
+            st.t[a] = upper ? t - 0.5 : t + 0.5;
+        } else {
+            st.c[a] = i0;
+            st.t[a] = t;
+        }
     }
-    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
-    // st[0] = potential; st[1 + a] = shift component a (staggered along a)
-    for (int f = 0; f < 4; ++f) {
-        const int a = f - 1;
-        const int ix = (a == 0) ? i1[0] : i0[0];
-        const int iy = (a == 1) ? i1[1] : i0[1];
-        const int iz = (a == 2) ? i1[2] : i0[2];
-        st[f].off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
-        st[f].c[0] = ix;
-        st[f].c[1] = iy;
-        st[f].c[2] = iz;
-        st[f].t[0] = a == 0 ? t1[0] : t0[0];
-        st[f].t[1] = a == 1 ? t1[1] : t0[1];
-        st[f].t[2] = a == 2 ? t1[2] : t0[2];
+    st.off = (size_t)st.c[0] + (size_t)st.c[1] * (size_t)G.pad + (size_t)st.c[2] * (size_t)G.pad * (size_t)G.pad;
+    return st;
+}
+
+__device__ __forceinline__ const double* field_grid64(const Grids& G, int f) {
+    return f == 0 ? G.phi64 : (f == 1 ? G.b64[0] : (f == 2 ? G.b64[1] : G.b64[2]));
+}
+
+// Interpolation outputs of one field at its stencil, in kOrders order: the
+// first 10 (value, gradient, Hessian), and with kThird the 10 third
+// derivatives.  FP64 grid rows; the z slices are a loop, which keeps the code
+// and the live values small.
+template <bool kThird>
+__device__ __forceinline__ void field_interp(const double* __restrict__ grid, size_t py, size_t pz,
+                                             const Stencil& st, double o[20]) {
+    constexpr int kOut = kThird ? 20 : 10;
+    constexpr int kOrd = kThird ? 4 : 3;
+    const Axis4 ax = cr_axis4(st.t[0]), ay = cr_axis4(st.t[1]), az = cr_axis4(st.t[2]);
+    for (int k = 0; k < kOut; ++k) o[k] = 0.0;
+    const double* base = grid + st.off;
+#pragma unroll 1
+    for (int c = 0; c < 4; ++c) {
+        double Y[4][4];  // y-pass sums [ox][oy], ox + oy < kOrd
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Y[i][j] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const Row4d r = load_row(base + b * py + c * pz);
+            double X[4];
+#pragma unroll
+            for (int ox = 0; ox < kOrd; ++ox)
+                X[ox] = ax.w[ox][0] * r.x + ax.w[ox][1] * r.y + ax.w[ox][2] * r.z + ax.w[ox][3] * r.w;
+#pragma unroll
+            for (int ox = 0; ox < kOrd; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < kOrd - ox; ++oy) Y[ox][oy] += ay.w[oy][b] * X[ox];
+        }
+        // the z weights of this slice, selected by constant indices
+        const double wz[4] = {az.w[0][0] * (c == 0) + az.w[0][1] * (c == 1) + az.w[0][2] * (c == 2) + az.w[0][3] * (c == 3),
+                              az.w[1][0] * (c == 0) + az.w[1][1] * (c == 1) + az.w[1][2] * (c == 2) + az.w[1][3] * (c == 3),
+                              az.w[2][0] * (c == 0) + az.w[2][1] * (c == 1) + az.w[2][2] * (c == 2) + az.w[2][3] * (c == 3),
+                              -3.0 * (c == 0) + 9.0 * (c == 1) - 9.0 * (c == 2) + 3.0 * (c == 3)};
+#pragma unroll
+        for (int k = 0; k < kOut; ++k) o[k] += wz[kOrders[k][2]] * Y[kOrders[k][0]][kOrders[k][1]];
     }
 }
 
-// Value, gradient and Hessian of the four fields at their stencils (FP64 grids).
-__device__ __forceinline__ void interp_fields(const Grids& G, const Stencil st[4], Interp<double> q[4]) {
-    const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
-    const double* grids[4] = {G.phi64, G.b64[0], G.b64[1], G.b64[2]};
-    for (int f = 0; f < 4; ++f) {
-        const Axis<double> wx = cr_axis<double>(st[f].t[0]), wy = cr_axis<double>(st[f].t[1]),
-                           wz = cr_axis<double>(st[f].t[2]);
-        q[f] = tricubic<double>(grids[f] + st[f].off, py, pz, wx, wy, wz);
-    }
-}
-
-// Right-hand side at stage state s (19, FP64), keeping the intermediates,
-// from the four fields' interpolants at s (interp_fields).  The third
-// derivatives the position adjoint needs are re-interpolated in rhs_vjp
-// rather than kept live from here.
-__device__ __forceinline__ void rhs_forward_from(const double* s, const Interp<double> qf[4], RhsTape& T,
+// Right-hand side at stage state s (19, FP64), keeping the intermediates, from
+// the four fields' interpolants at s: q[10 f + o], o = value, gradient (3),
+// Hessian (xx xy xz yy yz zz), read with stride `qs`.
+__device__ __forceinline__ void rhs_forward_from(const double* s, const double* q, int qs, RhsTape& T,
                                                  double dy[19]);
 
-// The same with the interpolations done here.
-__device__ __forceinline__ void rhs_forward(const Grids& G, const double* s, const Stencil st[4], RhsTape& T,
-                                            double dy[19]) {
-    Interp<double> q[4];
-    interp_fields(G, st, q);
-    rhs_forward_from(s, q, T, dy);
-}
-
-__device__ __forceinline__ void rhs_forward_from(const double* s, const Interp<double> qf[4], RhsTape& T,
+__device__ __forceinline__ void rhs_forward_from(const double* s, const double* q, int qs, RhsTape& T,
                                                  double dy[19]) {
     for (int i = 0; i < 3; ++i) {
         T.nd[i] = s[3 + i];
@@ -287,11 +346,13 @@ __device__ __forceinline__ void rhs_forward_from(const double* s, const Interp<d
         T.e2[i] = T.e2d[i] * T.r2;
     }
     for (int f = 0; f < 4; ++f) {
-        const Interp<double>& q = qf[f];
+        const double* qf = q + 10 * f * qs;
         const struct {
             double v, g[3];
-        } r = {q.v, {q.gx, q.gy, q.gz}};
-        const double H[3][3] = {{q.hxx, q.hxy, q.hxz}, {q.hxy, q.hyy, q.hyz}, {q.hxz, q.hyz, q.hzz}};
+        } r = {qf[0], {qf[qs], qf[2 * qs], qf[3 * qs]}};
+        const double hxx = qf[4 * qs], hxy = qf[5 * qs], hxz = qf[6 * qs], hyy = qf[7 * qs], hyz = qf[8 * qs],
+                     hzz = qf[9 * qs];
+        const double H[3][3] = {{hxx, hxy, hxz}, {hxy, hyy, hyz}, {hxz, hyz, hzz}};
         if (f == 0) {
             T.phi = r.v;
             for (int i = 0; i < 3; ++i) {
@@ -388,10 +449,15 @@ __device__ __forceinline__ void rhs_forward_from(const double* s, const Interp<d
     }
 }
 
-// Vector-Jacobian product of the right-hand side: given a = dL/d(dy) (19),
-// returns sbar = dL/ds (19) and scatters dL/d(grid) for the four fields.
-__device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], const RhsTape& T, const double a[19],
-                                        double* const gdst[4], double sbar[19]) {
+// Vector-Jacobian product of the right-hand side without its interpolations:
+// given the tape T and a = dL/d(dy) (19), returns sbar = dL/ds (19).  `fr`
+// points at the stage's per-field slots (field f, output o at
+// fr[(20 f + o) * stride], kOrders order): the position adjoint reads the
+// third derivatives there, and the adjoints of each field's interpolated
+// value, gradient and Hessian (value, gx, gy, gz, xx, xy + yx, xz + zx, yy,
+// yz + zy, zz: what scatter_taps takes) are written over its first 10 slots.
+__device__ __forceinline__ void rhs_vjp_bars(const RhsTape& T, const double a[19], gptr<double> fr, int64_t stride,
+                                             double sbar[19]) {
     const double* n = T.n;
     const double* e1 = T.e1;
     const double* e2 = T.e2;
@@ -590,18 +656,17 @@ __device__ __forceinline__ void rhs_vjp(const Grids& G, const Stencil st[4], con
             bar[9] = Hbb[aa][2][2];
         }
         // position: d(value)/dx = g, d(g_i)/dx_k = H_ik, d(h_u)/dx_k = third derivative
-        const Axis4 ax = cr_axis4(st[f].t[0]), ay = cr_axis4(st[f].t[1]), az = cr_axis4(st[f].t[2]);
-        const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
-        const float* grids[4] = {G.phi, G.b[0], G.b[1], G.b[2]};
-        const Interp3 r3 = tricubic3(grids[f] + st[f].off, py, pz, ax, ay, az);
+        const gptr<double> ff = fr + (int64_t)(20 * f) * stride;
+        double t3[10];
+        for (int u = 0; u < 10; ++u) t3[u] = ff[(10 + u) * stride];
         static const int hu[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
         for (int k = 0; k < 3; ++k) {
             double acc = bar[0] * gcur[k];
             for (int i = 0; i < 3; ++i) acc += bar[1 + i] * Hcur[i][k];
-            for (int u = 0; u < 6; ++u) acc += bar[4 + u] * r3.t[t3idx(hu[u][0], hu[u][1], k)];
+            for (int u = 0; u < 6; ++u) acc += bar[4 + u] * t3[t3idx(hu[u][0], hu[u][1], k)];
             xb[k] += acc;
         }
-        scatter_taps(gdst[f], st[f].c[0], st[f].c[1], st[f].c[2], py, pz, ax, ay, az, bar);
+        for (int o = 0; o < 10; ++o) ff[o * stride] = bar[o];
     }
 
     // normalisations: n = nd / |nd| etc.

@@ -494,260 +494,314 @@ __global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void batch_disp
 }
 
 // ------------------------------------------------------------- adjoint ----
-// One ray per lane, FP64.  Forward pass storing the state every K steps
-// (checkpoints), the six maps in FP64, then, segment by segment from the
-// last, recomputation of the segment's step states from its checkpoint and
-// the reverse sweep of each RK4 step (k4, k3, k2, k1 vector-Jacobian
-// products), which scatters dL/d(grid) into padded FP64 gradient grids.
-constexpr int kAdjBlock = 64;
+// The adjoint of the FP64 trace runs per chunk of rays as
+//   1. adj_forward_kernel<false>: the forward sweep, storing the state every K
+//      steps (checkpoints), the FP64 maps and dL/dy at the end;
+// and, per checkpoint segment from the last,
+//   2. adj_forward_kernel<true>: the segment's steps again from its
+//      checkpoint, storing every stage's state and each field's interpolation
+//      outputs up to the third derivatives (a "stage record");
+//   3. adj_reverse_kernel: the reverse sweep of the segment's RK4 steps over
+//      the stage records (vector-Jacobian products of the stages, pure
+//      algebra), writing each field's interpolation adjoints into the records;
+//   4. adj_scatter_kernel: the transpose of each record's 64-tap contraction,
+//      added to padded FP64 gradient grids.
+// Kernels 1 and 2 give a ray four lanes, one per field (potential, B_x, B_y,
+// B_z): the lanes interpolate their fields, exchange the 40 interpolants
+// through LDS and evaluate the right-hand side together; kernel 4 runs one
+// wavefront per 64 rays and (step, stage, field).
+constexpr int kAdjRays = 16;                    // rays per 64-lane workgroup of adj_forward_kernel
+constexpr int kAdjRec = 19 + 4 * 20;            // doubles per stage record: state, then 20 per field
+constexpr int64_t kAdjChunkBytes = int64_t(8) << 30;  // per-chunk buffers (records, checkpoints, dL/dy)
 
 struct AdjArgs {
     srt::Grids G;
-    int64_t nside, pix_begin, count;
+    int64_t nside, pix_begin;
     const int64_t* pix_list;
-    int n_steps, K;
-    double obs[3];
-    const double* map_grad;  // 6 x count, null = forward only
-    double* maps64;          // 6 x count, may be null
-    double* ckpt;            // count x (nseg + 1) x 19
-    double* work;            // count x ((K + 5) x 19 + 160): step and stage states, a stage RHS, interpolants
-    double* grad[4];         // padded gradient grids: potential, B_x, B_y, B_z
     const uint32_t* lane_slot;  // sorted lane -> slot map (srt::order_lanes), or null
+    int64_t lane0;              // first lane of this chunk
+    int64_t count;              // rays of this chunk
+    int64_t C;                  // stride of the chunk's per-ray buffers (component-major, ray fastest)
+    int64_t total;              // rays of the whole call: stride of maps64 and map_grad
+    int n_steps, K;
+    int seg, seg_steps;         // kernels 2-4: the segment and its step count
+    double obs[3];
+    const double* map_grad;  // 6 x total, null = forward only
+    double* maps64;          // 6 x total, may be null
+    double* ckpt;            // (nseg + 1) x 19 x C
+    double* lam;             // 19 x C: dL/dy at the start of the segment being reversed
+    double* rec;             // (4 K) x kAdjRec x C stage records
+    double* grad[4];         // padded gradient grids: potential, B_x, B_y, B_z
 };
 
-// FP64 right-hand side on a 19-vector (production kernel arithmetic)
-__device__ __forceinline__ void rhs19(const srt::Grids& G, const double* s, double* k) {
-    double ys[10];
-    for (int j = 0; j < 6; ++j) ys[j] = s[6 + j];
-    for (int j = 0; j < 4; ++j) ys[6 + j] = s[14 + j];
-    double kx[3];
-    srt::rhs<srt::FormFP64>(G, s, s + 3, s[12], s[13], ys, kx, k);
-    for (int i = 0; i < 3; ++i) k[i] = kx[i];
-    k[12] = -s[13];
+// Slot and RING pixel of chunk lane l; false for an invalid entry of a pixel list.
+__device__ __forceinline__ bool adj_ray(const AdjArgs& A, int64_t l, int64_t& slot, int64_t& pix) {
+    const int64_t lane = A.lane0 + l;
+    slot = A.lane_slot ? (int64_t)A.lane_slot[lane] : lane;
+    pix = A.pix_list ? A.pix_list[slot] : A.pix_begin + slot;
+    return pix >= 0 && pix < 12 * A.nside * A.nside;
 }
 
-__device__ __forceinline__ void rk4_step(const srt::Grids& G, double* y) {
-    double t[19], k[19], acc[19];
-    rhs19(G, y, acc);
-    for (int j = 0; j < 19; ++j) t[j] = y[j] + 0.5 * acc[j];
-    rhs19(G, t, k);
-    for (int j = 0; j < 19; ++j) {
-        t[j] = y[j] + 0.5 * k[j];
-        acc[j] += 2.0 * k[j];
-    }
-    rhs19(G, t, k);
-    for (int j = 0; j < 19; ++j) {
-        t[j] = y[j] + k[j];
-        acc[j] += 2.0 * k[j];
-    }
-    rhs19(G, t, k);
-    constexpr double sixth = 1.0 / 6.0;
-    for (int j = 0; j < 19; ++j) y[j] += (acc[j] + k[j]) * sixth;
-}
-
-// Per-lane vectors of the reverse sweep, in LDS (one column per lane, no
-// sharing, no barriers): dL/dy of the step being reversed, its running sum
-// over the stages, and the stage VJP's input dL/dk and output dL/ds.
-enum { kLam = 0, kYbar = 1, kKbar = 2, kSbar = 3 };
-__shared__ double adj_vec[4][19][kAdjBlock];
-
-// Per-lane buffers are component-major with the lane fastest: component j of
-// a lane's vector at p[j * stride].  The 40 interpolants of a stage (value,
-// gradient and Hessian of the four fields) are stored the same way, field f,
-// output o at component 10 f + o.
-constexpr int kInterpDoubles = 40;
-
-__device__ __forceinline__ void store_interps(double* q_out, int64_t stride, const srt::Interp<double> q[4]) {
-    const srt::adj::gptr<double> g = (srt::adj::gptr<double>)q_out;
-    for (int f = 0; f < 4; ++f) {
-        const double v[10] = {q[f].v, q[f].gx, q[f].gy, q[f].gz, q[f].hxx, q[f].hxy, q[f].hxz, q[f].hyy, q[f].hyz, q[f].hzz};
-        for (int o = 0; o < 10; ++o) g[(10 * f + o) * stride] = v[o];
-    }
-}
-
-__device__ __forceinline__ void load_interps(const double* q_in, int64_t stride, srt::Interp<double> q[4]) {
-    const srt::adj::gptr<const double> g = (srt::adj::gptr<const double>)q_in;
-    for (int f = 0; f < 4; ++f) {
-        double v[10];
-        for (int o = 0; o < 10; ++o) v[o] = g[(10 * f + o) * stride];
-        q[f] = srt::Interp<double>{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9]};
-    }
-}
-
-// Right-hand side k at stage state s (both component-major, `stride`), storing
-// the four fields' interpolants at s in q_out for that stage's VJP.  k may be
-// null (the last stage needs only the interpolants).  Not inlined, like
-// stage_vjp.
-__device__ __noinline__ void stage_rhs(const srt::Grids& G_in, const double* __restrict__ s_in, int64_t stride,
-                                       double* __restrict__ q_out, double* __restrict__ k_out) {
-    const srt::Grids G = G_in;
-    double s[19];
-    const srt::adj::gptr<const double> sg = (srt::adj::gptr<const double>)s_in;
-    for (int j = 0; j < 19; ++j) s[j] = sg[j * stride];
-    srt::adj::Stencil st[4];
-    srt::adj::locate_stencils(G, s, st);
-    srt::Interp<double> q[4];
-    srt::adj::interp_fields(G, st, q);
-    store_interps(q_out, stride, q);
-    if (k_out) {
-        srt::adj::RhsTape T;
-        double dy[19];
-        srt::adj::rhs_forward_from(s, q, T, dy);
-        const srt::adj::gptr<double> kg = (srt::adj::gptr<double>)k_out;
-        for (int j = 0; j < 19; ++j) kg[j * stride] = dy[j];
-    }
-}
-
-// dL/ds of one stage given dL/dk (adj_vec[kKbar]), into adj_vec[kSbar],
-// scattering the grid gradient.  The stage state is component j at
-// s[j * stride], its interpolants (stage_rhs) at q_in.  Not inlined: a call
-// boundary keeps the register allocation of the reverse sweep local.
-__device__ __noinline__ void stage_vjp(const AdjArgs& A, const double* __restrict__ s_in, const double* __restrict__ q_in,
-                                       int64_t stride) {
-    const int l = threadIdx.x;
-    double s[19], kbar[19], sbar[19];
-    const srt::adj::gptr<const double> sg = (srt::adj::gptr<const double>)s_in;
-    for (int j = 0; j < 19; ++j) {
-        s[j] = sg[j * stride];
-        kbar[j] = adj_vec[kKbar][j][l];
-    }
-    // the grids and gradient pointers, read once (A is not re-read after the
-    // gradient atomics, which the compiler cannot prove do not alias it)
-    const srt::Grids G = A.G;
-    double* const gd[4] = {A.grad[0], A.grad[1], A.grad[2], A.grad[3]};
-    srt::adj::Stencil st[4];
-    srt::adj::locate_stencils(G, s, st);
-    srt::Interp<double> q[4];
-    load_interps(q_in, stride, q);
-    srt::adj::RhsTape T;
-    double dy[19];
-    srt::adj::rhs_forward_from(s, q, T, dy);
-    srt::adj::rhs_vjp(G, st, T, kbar, gd, sbar);
-    for (int j = 0; j < 19; ++j) adj_vec[kSbar][j][l] = sbar[j];
-}
-
-__global__ __launch_bounds__(kAdjBlock) void adjoint_kernel(AdjArgs A) {
-    // lanes walk the rays in the order of A.lane_slot (compact sky patches, so
-    // a wavefront's stencils share cells and the gradient scatter stays in a
-    // small LDS tile); `lane` indexes the per-lane work buffers, `slot` the
-    // caller's arrays
-    const int64_t lane = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
-    if (lane >= A.count) return;
-    const int64_t slot = A.lane_slot ? (int64_t)A.lane_slot[lane] : lane;
-    const int64_t pix = A.pix_list ? A.pix_list[slot] : A.pix_begin + slot;
-    const int64_t npix = 12 * A.nside * A.nside;
-    if (pix < 0 || pix >= npix) {
-        if (A.maps64)
-            for (int m = 0; m < 6; ++m) A.maps64[m * A.count + slot] = __builtin_nan("");
+// Forward sweep (kRecord = false: the whole trace from the pixel's initial
+// state, storing checkpoints, maps and the final dL/dy) or one segment from
+// its checkpoint (kRecord = true: storing stage records).  Lane 4 r + f of the
+// workgroup serves ray r and field f.  The ray's state, its two stage states
+// and the 40 interpolants live in LDS; component j of a state is updated by
+// lane j mod 4 of the ray (each lane evaluates the whole right-hand side).
+template <bool kRecord>
+__global__ __launch_bounds__(64) void adj_forward_kernel(AdjArgs A) {
+    __shared__ double sy[19][kAdjRays], sbuf[2][19][kAdjRays], sq[40][kAdjRays];
+    const int r = threadIdx.x >> 2, f = threadIdx.x & 3;
+    const int64_t l = (int64_t)blockIdx.x * kAdjRays + r;
+    if (l >= A.count) return;  // the four lanes of a ray leave together
+    int64_t slot, pix;
+    if (!adj_ray(A, l, slot, pix)) {
+        if (!kRecord && A.maps64 && f == 0)
+            for (int m = 0; m < 6; ++m) A.maps64[m * A.total + slot] = __builtin_nan("");
         return;
     }
-    double z, s, ph;
-    srt::ring_pixel_direction(A.nside, pix, z, s, ph);
-    double sp, cp;
-    sincos(ph, &sp, &cp);
-    double y[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp, 0.0,
-                    0.0,      -1.0,     0.0,      0.0,     0.0,     0.0, 0.0};
+    const int64_t C = A.C;
     const int K = A.K;
-    const int nseg = (A.n_steps + K - 1) / K;
-    const int64_t C = A.count;
-    // checkpoints and segment states are component-major with the lane fastest
-    // (state q, component j of this lane at [(q * 19 + j) * C + lane]), so the
-    // lanes of a wavefront read and write them coalesced
-    double* const ck = A.ckpt + lane;
-    for (int sgi = 0; sgi < nseg; ++sgi) {
-        for (int j = 0; j < 19; ++j) ck[((int64_t)sgi * 19 + j) * C] = y[j];
-        const int m = min(K, A.n_steps - sgi * K);
-        for (int st = 0; st < m; ++st) rk4_step(A.G, y);
+    double z = 0, s = 0, ph = 0, sp = 0, cp = 0;
+    if (!kRecord) {
+        srt::ring_pixel_direction(A.nside, pix, z, s, ph);
+        sincos(ph, &sp, &cp);
+        const double init[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp,
+                                 0.0,      0.0,      -1.0,     0.0,      0.0,     0.0, 0.0,    0.0};
+        if (f == 0)
+            for (int j = 0; j < 19; ++j) sy[j][r] = init[j];
+    } else {
+        const double* ck = A.ckpt + (int64_t)A.seg * 19 * C + l;
+        for (int j = f; j < 19; j += 4) sy[j][r] = ck[j * C];
     }
-    for (int j = 0; j < 19; ++j) ck[((int64_t)nseg * 19 + j) * C] = y[j];
-
+    __syncthreads();
+    const size_t py = (size_t)A.G.pad, pz = (size_t)A.G.pad * (size_t)A.G.pad;
+    const double* grid = srt::adj::field_grid64(A.G, f);
+    const int steps = kRecord ? A.seg_steps : A.n_steps;
+    double acc[5];  // this lane's components of the RK4 stage sum
+    for (int step = 0; step < steps; ++step) {
+        if (!kRecord && step % K == 0) {
+            double* ck = A.ckpt + (int64_t)(step / K) * 19 * C + l;
+            for (int j = f; j < 19; j += 4) ck[j * C] = sy[j][r];
+        }
+        for (int stage = 0; stage < 4; ++stage) {
+            // stage state: y, then y + k1 / 2 in sbuf[0], y + k2 / 2 in sbuf[1], y + k3 in sbuf[0]
+            double(*S)[kAdjRays] = stage == 0 ? sy : sbuf[(stage - 1) & 1];
+            const double x[3] = {S[0][r], S[1][r], S[2][r]};
+            const srt::adj::Stencil st = srt::adj::field_stencil(A.G, x, f);
+            double o[20];
+            srt::adj::field_interp<kRecord>(grid, py, pz, st, o);
+            for (int k = 0; k < 10; ++k) sq[10 * f + k][r] = o[k];
+            if (kRecord) {
+                double* rq = A.rec + (int64_t)(step * 4 + stage) * kAdjRec * C + l;
+                for (int j = f; j < 19; j += 4) rq[j * C] = S[j][r];
+                for (int k = 0; k < 20; ++k) rq[(19 + 20 * f + k) * C] = o[k];
+            }
+            __syncthreads();  // every field's interpolants are in sq
+            double sv[19], dy[19];
+            for (int j = 0; j < 19; ++j) sv[j] = S[j][r];
+            srt::adj::RhsTape T;
+            srt::adj::rhs_forward_from(sv, &sq[0][r], kAdjRays, T, dy);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const int j = f + 4 * m;
+                if (j >= 19) break;
+                // dy[j] by constant indices (a runtime index would put dy in scratch)
+                const double d = f == 0 ? dy[4 * m] : (f == 1 ? dy[4 * m + 1] : (f == 2 ? dy[4 * m + 2] : dy[(4 * m + 3) % 19]));
+                if (stage == 0)
+                    acc[m] = d;
+                else if (stage < 3)
+                    acc[m] += 2.0 * d;
+                if (stage < 3)
+                    sbuf[stage & 1][j][r] = sy[j][r] + (stage == 2 ? 1.0 : 0.5) * d;
+                else
+                    sy[j][r] = sy[j][r] + (acc[m] + d) * (1.0 / 6.0);
+            }
+            __syncthreads();  // sq and S are read before they are written again
+        }
+    }
+    if (kRecord) return;
+    {
+        const int nseg = (A.n_steps + K - 1) / K;
+        double* ck = A.ckpt + (int64_t)nseg * 19 * C + l;
+        for (int j = f; j < 19; j += 4) ck[j * C] = sy[j][r];
+    }
+    if (f != 0) return;
+    double y[19];
+    for (int j = 0; j < 19; ++j) y[j] = sy[j][r];
     // maps (.cu:566-582) in FP64, and dL/dy at the end from dL/dmaps
     const double th[3] = {z * cp, z * sp, -s};
     const double p2[3] = {sp, -cp, 0.0};
     double d[3] = {y[0] - A.obs[0], y[1] - A.obs[1], y[2] - A.obs[2]};
     const double dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    double u[3] = {d[0] / dn, d[1] / dn, d[2] / dn};
-    const double r[3] = {s * cp, s * sp, z};
+    const double u[3] = {d[0] / dn, d[1] / dn, d[2] / dn};
+    const double rr[3] = {s * cp, s * sp, z};
     double dl1 = 0, dl2 = 0;
     for (int i = 0; i < 3; ++i) {
-        dl1 += (u[i] - r[i]) * th[i];
-        dl2 += (u[i] - r[i]) * p2[i];
+        dl1 += (u[i] - rr[i]) * th[i];
+        dl2 += (u[i] - rr[i]) * p2[i];
     }
+    const int64_t T = A.total;
     if (A.maps64) {
-        A.maps64[0 * C + slot] = y[12];
-        A.maps64[1 * C + slot] = y[16];
-        A.maps64[2 * C + slot] = y[17];
-        A.maps64[3 * C + slot] = y[18];
-        A.maps64[4 * C + slot] = dl1;
-        A.maps64[5 * C + slot] = dl2;
+        A.maps64[0 * T + slot] = y[12];
+        A.maps64[1 * T + slot] = y[16];
+        A.maps64[2 * T + slot] = y[17];
+        A.maps64[3 * T + slot] = y[18];
+        A.maps64[4 * T + slot] = dl1;
+        A.maps64[5 * T + slot] = dl2;
     }
     if (!A.map_grad) return;
-    const int l = threadIdx.x;
-    double(*const vec)[19][kAdjBlock] = adj_vec;
-    for (int j = 0; j < 19; ++j) vec[kLam][j][l] = 0.0;
-    const double mb[6] = {A.map_grad[0 * C + slot], A.map_grad[1 * C + slot], A.map_grad[2 * C + slot],
-                          A.map_grad[3 * C + slot], A.map_grad[4 * C + slot], A.map_grad[5 * C + slot]};
-    vec[kLam][12][l] = mb[0];
-    vec[kLam][16][l] = mb[1];
-    vec[kLam][17][l] = mb[2];
-    vec[kLam][18][l] = mb[3];
-    {
-        // d(defl)/dx = (1 - u u^T) / |d| applied to mb4 theta-hat + mb5 p2
-        double w[3], uw = 0;
-        for (int i = 0; i < 3; ++i) {
-            w[i] = mb[4] * th[i] + mb[5] * p2[i];
-            uw += u[i] * w[i];
-        }
-        for (int i = 0; i < 3; ++i) vec[kLam][i][l] = (w[i] - u[i] * uw) / dn;
+    double lam[19];
+    for (int j = 0; j < 19; ++j) lam[j] = 0.0;
+    const double mb[6] = {A.map_grad[0 * T + slot], A.map_grad[1 * T + slot], A.map_grad[2 * T + slot],
+                          A.map_grad[3 * T + slot], A.map_grad[4 * T + slot], A.map_grad[5 * T + slot]};
+    lam[12] = mb[0];
+    lam[16] = mb[1];
+    lam[17] = mb[2];
+    lam[18] = mb[3];
+    // d(defl)/dx = (1 - u u^T) / |d| applied to mb4 theta-hat + mb5 p2
+    double w[3], uw = 0;
+    for (int i = 0; i < 3; ++i) {
+        w[i] = mb[4] * th[i] + mb[5] * p2[i];
+        uw += u[i] * w[i];
     }
+    for (int i = 0; i < 3; ++i) lam[i] = (w[i] - u[i] * uw) / dn;
+    for (int j = 0; j < 19; ++j) A.lam[j * C + l] = lam[j];
+}
 
-    // K step states, then 4 stage states, then one stage right-hand side, then
-    // the 4 stages' interpolants, component-major like the checkpoints
-    double* const wk = A.work + lane;
-    auto W = [&](int q, int j) -> double& { return wk[((int64_t)q * 19 + j) * C]; };
-    auto IQ = [&](int stage) -> double* { return wk + ((int64_t)(K + 5) * 19 + stage * kInterpDoubles) * C; };
-    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-        for (int j = 0; j < 19; ++j) y[j] = ck[((int64_t)sgi * 19 + j) * C];
-        const int m = min(K, A.n_steps - sgi * K);
-        for (int st = 0; st < m; ++st) {
-            for (int j = 0; j < 19; ++j) W(st, j) = y[j];
-            rk4_step(A.G, y);
+// Per-lane vectors of the reverse sweep, in LDS (one column per lane, no
+// sharing, no barriers): dL/dy of the step being reversed, its running sum
+// over the stages, and the stage VJP's input dL/dk and output dL/ds.
+constexpr int kAdjRevBlock = 64;
+enum { kLam = 0, kYbar = 1, kKbar = 2, kSbar = 3 };
+__shared__ double adj_vec[4][19][kAdjRevBlock];
+
+// dL/ds of one stage given dL/dk (adj_vec[kKbar]), into adj_vec[kSbar], from
+// the stage record at `rq` (component c at rq[c * C]); the record's per-field
+// value, gradient and Hessian slots receive their adjoints.  Not inlined: a
+// call boundary keeps the register allocation of the sweep local.
+__device__ __noinline__ void stage_vjp(double* __restrict__ rq_in, int64_t C) {
+    const int l = threadIdx.x;
+    const srt::adj::gptr<double> rq = (srt::adj::gptr<double>)rq_in;
+    double sv[19], kbar[19], sbar[19], q[40];
+    for (int j = 0; j < 19; ++j) {
+        sv[j] = rq[j * C];
+        kbar[j] = adj_vec[kKbar][j][l];
+    }
+    for (int f = 0; f < 4; ++f)
+        for (int k = 0; k < 10; ++k) q[10 * f + k] = rq[(19 + 20 * f + k) * C];
+    srt::adj::RhsTape T;
+    double dy[19];
+    srt::adj::rhs_forward_from(sv, q, 1, T, dy);
+    srt::adj::rhs_vjp_bars(T, kbar, rq + 19 * C, C, sbar);
+    for (int j = 0; j < 19; ++j) adj_vec[kSbar][j][l] = sbar[j];
+}
+
+// Reverse sweep of one segment, one lane per ray: for each step from the
+// last, the transpose of y' = y + (k1 + 2 k2 + 2 k3 + k4) / 6 through the
+// four stages' VJPs (k4, k3, k2, k1).
+__global__ __launch_bounds__(kAdjRevBlock) void adj_reverse_kernel(AdjArgs A) {
+    const int64_t l = (int64_t)blockIdx.x * kAdjRevBlock + threadIdx.x;
+    if (l >= A.count) return;
+    int64_t slot, pix;
+    if (!adj_ray(A, l, slot, pix)) return;
+    const int t = threadIdx.x;
+    const int64_t C = A.C;
+    double(*const vec)[19][kAdjRevBlock] = adj_vec;
+    for (int j = 0; j < 19; ++j) vec[kLam][j][t] = A.lam[j * C + l];
+    for (int st = A.seg_steps - 1; st >= 0; --st) {
+        double* const rq = A.rec + (int64_t)(st * 4) * kAdjRec * C + l;
+        const int64_t stage_stride = (int64_t)kAdjRec * C;
+        for (int j = 0; j < 19; ++j) {
+            vec[kYbar][j][t] = vec[kLam][j][t];
+            vec[kKbar][j][t] = vec[kLam][j][t] * (1.0 / 6.0);
         }
-        for (int st = m - 1; st >= 0; --st) {
-            // stage states of this step: s1 = y, s2 = y + k1/2, s3 = y + k2/2,
-            // s4 = y + k3, with each stage's interpolants kept for its VJP
-            {
-                for (int j = 0; j < 19; ++j) W(K, j) = W(st, j);
-                const double c[3] = {0.5, 0.5, 1.0};
-                for (int i = 0; i < 3; ++i) {
-                    stage_rhs(A.G, &W(K + i, 0), C, IQ(i), &W(K + 4, 0));
-                    for (int j = 0; j < 19; ++j) W(K + 1 + i, j) = W(st, j) + c[i] * W(K + 4, j);
-                }
-                stage_rhs(A.G, &W(K + 3, 0), C, IQ(3), nullptr);
-            }
-            // reverse sweep: y' = y + (k1 + 2 k2 + 2 k3 + k4) / 6
-            for (int j = 0; j < 19; ++j) {
-                vec[kYbar][j][l] = vec[kLam][j][l];
-                vec[kKbar][j][l] = vec[kLam][j][l] * (1.0 / 6.0);
-            }
-            stage_vjp(A, &W(K + 3, 0), IQ(3), C);  // s4 = y + k3
-            for (int j = 0; j < 19; ++j) {
-                vec[kYbar][j][l] += vec[kSbar][j][l];
-                vec[kKbar][j][l] = vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 3.0);
-            }
-            stage_vjp(A, &W(K + 2, 0), IQ(2), C);  // s3 = y + k2 / 2
-            for (int j = 0; j < 19; ++j) {
-                vec[kYbar][j][l] += vec[kSbar][j][l];
-                vec[kKbar][j][l] = 0.5 * vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 3.0);
-            }
-            stage_vjp(A, &W(K + 1, 0), IQ(1), C);  // s2 = y + k1 / 2
-            for (int j = 0; j < 19; ++j) {
-                vec[kYbar][j][l] += vec[kSbar][j][l];
-                vec[kKbar][j][l] = 0.5 * vec[kSbar][j][l] + vec[kLam][j][l] * (1.0 / 6.0);
-            }
-            stage_vjp(A, &W(K, 0), IQ(0), C);  // s1 = y
-            for (int j = 0; j < 19; ++j) vec[kLam][j][l] = vec[kYbar][j][l] + vec[kSbar][j][l];
+        stage_vjp(rq + 3 * stage_stride, C);  // s4 = y + k3
+        for (int j = 0; j < 19; ++j) {
+            vec[kYbar][j][t] += vec[kSbar][j][t];
+            vec[kKbar][j][t] = vec[kSbar][j][t] + vec[kLam][j][t] * (1.0 / 3.0);
         }
+        stage_vjp(rq + 2 * stage_stride, C);  // s3 = y + k2 / 2
+        for (int j = 0; j < 19; ++j) {
+            vec[kYbar][j][t] += vec[kSbar][j][t];
+            vec[kKbar][j][t] = 0.5 * vec[kSbar][j][t] + vec[kLam][j][t] * (1.0 / 3.0);
+        }
+        stage_vjp(rq + 1 * stage_stride, C);  // s2 = y + k1 / 2
+        for (int j = 0; j < 19; ++j) {
+            vec[kYbar][j][t] += vec[kSbar][j][t];
+            vec[kKbar][j][t] = 0.5 * vec[kSbar][j][t] + vec[kLam][j][t] * (1.0 / 6.0);
+        }
+        stage_vjp(rq, C);  // s1 = y
+        for (int j = 0; j < 19; ++j) vec[kLam][j][t] = vec[kYbar][j][t] + vec[kSbar][j][t];
+    }
+    for (int j = 0; j < 19; ++j) A.lam[j * C + l] = vec[kLam][j][t];
+}
+
+// Grid gradient of one segment's interpolation adjoints.  Workgroup b handles
+// the 64 rays of tile b mod tiles for job b / tiles = (step, field): the
+// records of the step's 4 stages of one field, whose stencils overlap (a ray
+// moves about one cell per step), are summed in one LDS tile
+// over the box of all of them (in z slabs that fit the tile) and flushed with
+// one global atomic per touched cell; only a box wider than the tile in one
+// z plane (rays across the periodic wrap) takes each record on its own.  Consecutive workgroups take neighbouring sky patches of
+// one job.  The stencils are recomputed from the recorded stage positions
+// exactly as the recording kernel located them.
+__global__ __launch_bounds__(64) void adj_scatter_kernel(AdjArgs A) {
+    const int64_t tiles = (A.count + 63) / 64;
+    const int64_t job = blockIdx.x / tiles, tile = blockIdx.x - job * tiles;
+    const int64_t l = tile * 64 + threadIdx.x;
+    if (l >= A.count) return;
+    int64_t slot, pix;
+    if (!adj_ray(A, l, slot, pix)) return;
+    const int f = (int)(job & 3);
+    const int s0 = (int)(job >> 2), s1 = s0 + 1;
+    const int64_t C = A.C;
+    const size_t py = (size_t)A.G.pad, pz = (size_t)A.G.pad * (size_t)A.G.pad;
+    double* const gdst = A.grad[f];
+    auto record = [&](int q) {
+        return (srt::adj::gptr<const double>)(A.rec + (int64_t)q * kAdjRec * C + l);
+    };
+    auto stencil = [&](int q) {
+        const srt::adj::gptr<const double> rq = record(q);
+        const double x[3] = {rq[0], rq[C], rq[2 * C]};
+        return srt::adj::field_stencil(A.G, x, f);
+    };
+    auto taps = [&](int q, const srt::adj::Stencil& st, auto&& add) {
+        const srt::adj::gptr<const double> rq = record(q);
+        double bar[10];
+        for (int k = 0; k < 10; ++k) bar[k] = rq[(19 + 20 * f + k) * C];
+        const srt::adj::Axis4 ax = srt::adj::cr_axis4(st.t[0]), ay = srt::adj::cr_axis4(st.t[1]),
+                              az = srt::adj::cr_axis4(st.t[2]);
+        add(st, ax, ay, az, bar);
+    };
+    int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+    for (int q = 4 * s0; q < 4 * s1; ++q) {
+        const srt::adj::Stencil st = stencil(q);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = min(lo[k], st.c[k]);
+            hi[k] = max(hi[k], st.c[k]);
+        }
+    }
+    const srt::adj::TileBox B = srt::adj::wave_box(lo, hi);
+    const int dz = srt::adj::slab_planes(B);
+    if (dz > 0) {
+        for (int zlo = 0; zlo < B.nz; zlo += dz) {
+            const srt::adj::TileBox P = srt::adj::slab(B, zlo, dz);
+            srt::adj::tile_clear(P.cells());
+            for (int q = 4 * s0; q < 4 * s1; ++q) {
+                const srt::adj::Stencil st = stencil(q);
+                if (st.c[2] + 3 < P.z0 || st.c[2] >= P.z0 + P.nz) continue;  // no tap in this slab
+                taps(q, st, [&](const srt::adj::Stencil& s, const auto& ax, const auto& ay, const auto& az,
+                                const double* bar) { srt::adj::tile_add(P, s.c[0], s.c[1], s.c[2], ax, ay, az, bar); });
+            }
+            srt::adj::tile_flush(gdst, P, py, pz);
+        }
+    } else {
+        for (int q = 4 * s0; q < 4 * s1; ++q)
+            taps(q, stencil(q), [&](const srt::adj::Stencil& st, const auto& ax, const auto& ay, const auto& az,
+                                    const double* bar) {
+                srt::adj::scatter_taps(gdst, st.c[0], st.c[1], st.c[2], py, pz, ax, ay, az, bar);
+            });
     }
 }
 
@@ -1350,17 +1404,21 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     if (map_grad && (!grad_phi || !grad_shift_xyz))
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: gradient outputs are null");
     if (!map_grad && !maps64) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: nothing to compute");
-    // one launch, whose lanes (grid size times workgroup size) must stay below 2^32
-    if ((count + kAdjBlock - 1) / kAdjBlock * kAdjBlock >= (int64_t(1) << 32))
-        return fail(SRT_ERR_UNSUPPORTED, "srt_trace_adjoint: more rays than one launch holds (2^32 lanes)");
+    // at most 2^32 - 64 rays (rounded to whole wavefronts, below 2^32 lanes; the
+    // launches themselves are split into chunks far smaller than that)
+    if ((count + 63) / 64 * 64 >= (int64_t(1) << 32))
+        return fail(SRT_ERR_UNSUPPORTED, "srt_trace_adjoint: more rays than one call holds (2^32 lanes)");
     SRT_HIP(hipSetDevice(ctx->device));
     const int K = std::min(checkpoint_every, n_steps);
     const int nseg = (n_steps + K - 1) / K;
     const size_t P = (size_t)ctx->pad, pvox = P * P * P;
-    const size_t c = (size_t)count;
-    const size_t n_ck = c * (size_t)(nseg + 1) * 19, n_wk = c * ((size_t)(K + 5) * 19 + 4 * kInterpDoubles),
-                 n_gr = map_grad ? 4 * pvox : 0;
-    const size_t need = (n_ck + n_wk + n_gr) * sizeof(double);
+    // rays per chunk: the chunk's stage records, checkpoints and dL/dy within kAdjChunkBytes
+    const int64_t per_ray = (int64_t)sizeof(double) * ((int64_t)4 * K * kAdjRec + (int64_t)(nseg + 1) * 19 + 19);
+    int64_t chunk = std::max<int64_t>(64, kAdjChunkBytes / per_ray / 64 * 64);
+    chunk = std::min<int64_t>(chunk, (std::max<int64_t>(count, 1) + 63) / 64 * 64);
+    const size_t n_ck = (size_t)chunk * (size_t)(nseg + 1) * 19, n_lam = (size_t)chunk * 19,
+                 n_rec = map_grad ? (size_t)chunk * (size_t)(4 * K) * kAdjRec : 0, n_gr = map_grad ? 4 * pvox : 0;
+    const size_t need = (n_ck + n_lam + n_rec + n_gr) * sizeof(double);
     if (need > ctx->adj_bytes) {
         SRT_HIP(hipStreamSynchronize((hipStream_t)stream));
         if (ctx->adj_buf) SRT_HIP(hipFree(ctx->adj_buf));
@@ -1384,8 +1442,9 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     set_records(ctx, A.G);
     A.nside = nside;
     A.pix_begin = pix_begin;
-    A.count = count;
     A.pix_list = pix_list;
+    A.total = count;
+    A.C = chunk;
     A.n_steps = n_steps;
     A.K = K;
     for (int i = 0; i < 3; ++i) A.obs[i] = observer ? observer[i] : 0.5 * (double)ctx->n;
@@ -1393,8 +1452,9 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     A.maps64 = maps64;
     A.lane_slot = nullptr;
     A.ckpt = ctx->adj_buf;
-    A.work = ctx->adj_buf + n_ck;
-    for (int f = 0; f < 4; ++f) A.grad[f] = map_grad ? ctx->adj_buf + n_ck + n_wk + f * pvox : nullptr;
+    A.lam = A.ckpt + n_ck;
+    A.rec = map_grad ? A.lam + n_lam : nullptr;
+    for (int f = 0; f < 4; ++f) A.grad[f] = map_grad ? A.lam + n_lam + n_rec + f * pvox : nullptr;
     if (map_grad) SRT_HIP(hipMemsetAsync(A.grad[0], 0, 4 * pvox * sizeof(double), st));
     // rays in NESTED order of their pixels (srt::order_lanes); the results do
     // not depend on it beyond the summation order of the gradient atomics
@@ -1417,10 +1477,22 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     if (A.lane_slot)
         SRT_HIP(srt::order_lanes(nside, pix_begin, count, pix_list, static_cast<uint32_t*>(ctx->order_buf),
                                  static_cast<char*>(ctx->order_buf) + map_bytes, ctx->order_bytes - map_bytes, st));
-    if (count > 0) {
-        const int64_t blocks = (count + kAdjBlock - 1) / kAdjBlock;
-        hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, A);
+    for (int64_t l0 = 0; l0 < count; l0 += chunk) {
+        A.lane0 = l0;
+        A.count = std::min(chunk, count - l0);
+        const unsigned fwd_blocks = (unsigned)((A.count + kAdjRays - 1) / kAdjRays);
+        hipLaunchKernelGGL(adj_forward_kernel<false>, dim3(fwd_blocks), dim3(64), 0, st, A);
         SRT_HIP(hipGetLastError());
+        if (!map_grad) continue;
+        const int64_t tiles = (A.count + 63) / 64;
+        for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+            A.seg = sgi;
+            A.seg_steps = std::min(K, n_steps - sgi * K);
+            hipLaunchKernelGGL(adj_forward_kernel<true>, dim3(fwd_blocks), dim3(64), 0, st, A);
+            hipLaunchKernelGGL(adj_reverse_kernel, dim3((unsigned)tiles), dim3(kAdjRevBlock), 0, st, A);
+            hipLaunchKernelGGL(adj_scatter_kernel, dim3((unsigned)(tiles * 4 * A.seg_steps)), dim3(64), 0, st, A);
+            SRT_HIP(hipGetLastError());
+        }
     }
     SRT_HIP(hipEventRecord(ctx->ev_stop, st));
     ctx->timed = true;
