@@ -57,15 +57,13 @@ constexpr signed char kOrders[20][3] = {
     {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
 
 // index of the third derivative d3/(dx_i dx_j dx_k) among the ten third derivatives (kOrders 10..19)
-__device__ __forceinline__ int t3idx(int i, int j, int k) {
-    // sort i <= j <= k
-    if (i > j) { const int s = i; i = j; j = s; }
-    if (j > k) { const int s = j; j = k; k = s; }
-    if (i > j) { const int s = i; i = j; j = s; }
-    // xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
-    static const signed char tab[3][3][3] = {
+constexpr int t3idx(int i, int j, int k) {
+    // sort i <= j <= k, then xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+    const int a = i < j ? i : j, b = i < j ? j : i;  // a <= b
+    const int lo = a < k ? a : k, hi = b > k ? b : k, mid = i + j + k - lo - hi;
+    constexpr signed char tab[3][3][3] = {
         {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}, {{1, 3, 4}, {3, 6, 7}, {4, 7, 8}}, {{2, 4, 5}, {4, 7, 8}, {5, 8, 9}}};
-    return tab[i][j][k];
+    return tab[lo][mid][hi];
 }
 
 __device__ __forceinline__ int h6(int i, int j) {
@@ -355,6 +353,7 @@ __device__ __forceinline__ void rhs_forward_from(const double* s, const double* 
         T.e1[i] = T.e1d[i] * T.r1;
         T.e2[i] = T.e2d[i] * T.r2;
     }
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
         const double* qf = q + 10 * f * qs;
         const struct {
@@ -588,6 +587,7 @@ __device__ __forceinline__ void rhs_vjp_bars(const RhsTape& T, const double a[19
         Nb[i] += 0.25 * gamb * n[i];
     }
     // shift couplings N, E1, E2, T, L
+#pragma unroll
     for (int aa = 0; aa < 3; ++aa) {
         const double* gb = T.gb[aa];
         double ng = 0, e1g = 0, e2g = 0;
@@ -633,6 +633,7 @@ __device__ __forceinline__ void rhs_vjp_bars(const RhsTape& T, const double a[19
 
     // interpolation adjoints: field f has (value, gradient, Hessian) adjoints
     double xb[3] = {0, 0, 0};
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
         double bar[10];
         double gcur[3];
@@ -669,10 +670,13 @@ __device__ __forceinline__ void rhs_vjp_bars(const RhsTape& T, const double a[19
         const gptr<double> ff = fr + (int64_t)(20 * f) * stride;
         double t3[10];
         for (int u = 0; u < 10; ++u) t3[u] = ff[(10 + u) * stride];
-        static const int hu[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        constexpr int hu[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             double acc = bar[0] * gcur[k];
+#pragma unroll
             for (int i = 0; i < 3; ++i) acc += bar[1 + i] * Hcur[i][k];
+#pragma unroll
             for (int u = 0; u < 6; ++u) acc += bar[4 + u] * t3[t3idx(hu[u][0], hu[u][1], k)];
             xb[k] += acc;
         }
