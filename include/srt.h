@@ -199,14 +199,15 @@ int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t c
  *   if map_grad (6 * count doubles, dL/d(map), map-major) is not null:
  *     grad_phi       (nx*ny*nz doubles)   dL/d(potential grid value),
  *     grad_shift_xyz (nx*ny*nz*3 doubles) dL/d(shift grid value), interleaved
- *   by a reverse sweep that recomputes each checkpoint segment's states,
+ *   by a reverse sweep that recomputes each checkpoint segment's stages,
  *   propagates dL/dy through RK4 steps, right-hand sides and tricubic
  *   interpolations, and adds the tap gradients with FP64 atomics (summed per
  *   wavefront in LDS first; the summation order, and so the last bits of the
  *   gradients, may change from call to call).
- * Rays go to lanes in NESTED order of their pixels, as in srt_trace.
- * All pointers are device pointers; asynchronous on `stream`.  Work buffers
- * (checkpoints, segment states, padded gradient grids, the lane map and its
+ * Rays go to lanes in NESTED order of their pixels, as in srt_trace, and are
+ * processed in chunks whose work buffers fit 8 GiB.  All pointers are device
+ * pointers; asynchronous on `stream`.  Work buffers (checkpoints, one
+ * segment's stage records, dL/dy, padded gradient grids, the lane map and its
  * sort space) are owned by the context and reused: overlapping calls on one
  * context must share a stream.  The gradient is with respect to the grid values the
  * context holds (the binary16-quantised ones for SRT_SHIFT_F16_SCALED). */
