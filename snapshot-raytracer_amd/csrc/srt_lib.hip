@@ -525,6 +525,7 @@ struct AdjArgs {
     int64_t total;              // rays of the whole call: stride of maps64 and map_grad
     int n_steps, K;
     int seg, seg_steps;         // kernels 2-4: the segment and its step count
+    int rec_from;               // kernel 1: the first step whose stages it records (the last segment's)
     double obs[3];
     const double* map_grad;  // 6 x total, null = forward only
     double* maps64;          // 6 x total, may be null
@@ -543,13 +544,15 @@ __device__ __forceinline__ bool adj_ray(const AdjArgs& A, int64_t l, int64_t& sl
 }
 
 // Forward sweep (kRecord = false: the whole trace from the pixel's initial
-// state, storing checkpoints, maps and the final dL/dy) or one segment from
-// its checkpoint (kRecord = true: storing stage records).  Lane 4 r + f of the
+// state, storing checkpoints, maps and the final dL/dy, and the stage records
+// of the steps from A.rec_from on: the last segment's, which the reverse
+// sweep needs first) or one segment from its checkpoint (kRecord = true:
+// storing its stage records).  Lane 4 r + f of the
 // workgroup serves ray r and field f.  The ray's state, its two stage states
 // and the 40 interpolants live in LDS; component j of a state is updated by
 // lane j mod 4 of the ray (each lane evaluates the whole right-hand side).
 template <bool kRecord>
-__global__ __launch_bounds__(64) void adj_forward_kernel(AdjArgs A) {
+__global__ __launch_bounds__(64, 2) void adj_forward_kernel(AdjArgs A) {
     __shared__ double sy[19][kAdjRays], sbuf[2][19][kAdjRays], sq[40][kAdjRays];
     const int r = threadIdx.x >> 2, f = threadIdx.x & 3;
     const int64_t l = (int64_t)blockIdx.x * kAdjRays + r;
@@ -590,10 +593,15 @@ __global__ __launch_bounds__(64) void adj_forward_kernel(AdjArgs A) {
             const double x[3] = {S[0][r], S[1][r], S[2][r]};
             const srt::adj::Stencil st = srt::adj::field_stencil(A.G, x, f);
             double o[20];
-            srt::adj::field_interp<kRecord>(grid, py, pz, st, o);
+            const bool record = kRecord || step >= A.rec_from;
+            if (record)
+                srt::adj::field_interp<true>(grid, py, pz, st, o);
+            else
+                srt::adj::field_interp<false>(grid, py, pz, st, o);
             for (int k = 0; k < 10; ++k) sq[10 * f + k][r] = o[k];
-            if (kRecord) {
-                double* rq = A.rec + (int64_t)(step * 4 + stage) * kAdjRec * C + l;
+            if (record) {
+                const int rs = kRecord ? step : step - A.rec_from;
+                double* rq = A.rec + (int64_t)(rs * 4 + stage) * kAdjRec * C + l;
                 for (int j = f; j < 19; j += 4) rq[j * C] = S[j][r];
                 for (int k = 0; k < 20; ++k) rq[(19 + 20 * f + k) * C] = o[k];
             }
@@ -1451,6 +1459,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     A.map_grad = map_grad;
     A.maps64 = maps64;
     A.lane_slot = nullptr;
+    A.rec_from = map_grad ? (nseg - 1) * K : n_steps;  // the forward sweep records the last segment
     A.ckpt = ctx->adj_buf;
     A.lam = A.ckpt + n_ck;
     A.rec = map_grad ? A.lam + n_lam : nullptr;
@@ -1488,7 +1497,8 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
         for (int sgi = nseg - 1; sgi >= 0; --sgi) {
             A.seg = sgi;
             A.seg_steps = std::min(K, n_steps - sgi * K);
-            hipLaunchKernelGGL(adj_forward_kernel<true>, dim3(fwd_blocks), dim3(64), 0, st, A);
+            if (sgi < nseg - 1)  // the forward sweep left the last segment's records
+                hipLaunchKernelGGL(adj_forward_kernel<true>, dim3(fwd_blocks), dim3(64), 0, st, A);
             hipLaunchKernelGGL(adj_reverse_kernel, dim3((unsigned)tiles), dim3(kAdjRevBlock), 0, st, A);
             hipLaunchKernelGGL(adj_scatter_kernel, dim3((unsigned)(tiles * 4 * A.seg_steps)), dim3(64), 0, st, A);
             SRT_HIP(hipGetLastError());
