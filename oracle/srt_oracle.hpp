@@ -26,6 +26,12 @@
 //              GPU code does.  State and RK4 stay double (.cu:513-549).
 // A third instantiation with an operation-counting scalar type counts flops.
 //
+// Pins (tests/): every function here is checked against closed forms, brute
+// force or first-order physics, except two terms of rhs, which are checked
+// only against an independent tensor-form restatement of the same equations
+// (test_oracle_physics.py): L (nLaplaceB, entering only D'') and the
+// antisymmetric part of E1, E2.  For those two terms parity is unpinned.
+//
 // Periodic wrap: the reference forms (i + o + 20 n) % n (.cu:241-256), valid
 // while a ray stays within 20 boxes on the negative side.  This restatement
 // uses the true modulo, which agrees with the reference wherever the
