@@ -95,7 +95,9 @@ typedef enum {
  * SRT_XCD_GROUPS (workgroup grouping of trace launches, default 8) and
  * SRT_MAX_LAUNCH_RAYS (rays per trace launch, default and maximum 2^31: one
  * launch may not hold 2^32 lanes, so a larger trace is split into several
- * launches; smaller values exercise the split). */
+ * launches; smaller values exercise the split) and SRT_ADJ_CHUNK_RAYS (largest
+ * ray chunk of srt_trace_adjoint, default: what fits its 8 GiB of work
+ * buffers; smaller values exercise the chunk loop). */
 int srt_create(int device, srt_context** ctx);
 
 /* Frees every device buffer of the context (.cu:917-920, 989-998). */

@@ -940,6 +940,11 @@ struct srt_context {
     // context creation (rounded down to a multiple of 256), so that tests can
     // exercise the split with small traces
     int64_t max_launch_rays = int64_t(1) << 31;
+    // largest ray chunk of an adjoint call (0 = as many rays as the chunk's
+    // buffers allow in kAdjChunkBytes); from SRT_ADJ_CHUNK_RAYS at context
+    // creation (rounded down to a multiple of 64), so that tests can run the
+    // chunk loop with small traces
+    int64_t adj_chunk_rays = 0;
     // sorted lane order of partial-sky NESTED traces: lane -> slot map and the
     // sort's work space, grown on demand
     void* order_buf = nullptr;
@@ -1116,6 +1121,10 @@ int srt_create(int device, srt_context** ctx) {
     if (const char* m = std::getenv("SRT_MAX_ALLOC_BYTES")) {
         const long long v = std::atoll(m);
         if (v > 0) c->max_alloc = (size_t)v;
+    }
+    if (const char* m = std::getenv("SRT_ADJ_CHUNK_RAYS")) {
+        const long long v = std::atoll(m) / 64 * 64;
+        if (v >= 64) c->adj_chunk_rays = v;
     }
     if (const char* m = std::getenv("SRT_MAX_LAUNCH_RAYS")) {
         const long long v = std::atoll(m) / 256 * 256;
@@ -1423,6 +1432,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     // rays per chunk: the chunk's stage records, checkpoints and dL/dy within kAdjChunkBytes
     const int64_t per_ray = (int64_t)sizeof(double) * ((int64_t)4 * K * kAdjRec + (int64_t)(nseg + 1) * 19 + 19);
     int64_t chunk = std::max<int64_t>(64, kAdjChunkBytes / per_ray / 64 * 64);
+    if (ctx->adj_chunk_rays) chunk = std::min(chunk, ctx->adj_chunk_rays);
     chunk = std::min<int64_t>(chunk, (std::max<int64_t>(count, 1) + 63) / 64 * 64);
     const size_t n_ck = (size_t)chunk * (size_t)(nseg + 1) * 19, n_lam = (size_t)chunk * 19,
                  n_rec = map_grad ? (size_t)chunk * (size_t)(4 * K) * kAdjRec : 0, n_gr = map_grad ? 4 * pvox : 0;

@@ -258,3 +258,34 @@ def test_adjoint_rejects_launches_of_2_to_the_32_lanes(srt, setup):
         assert b"2^32" in L.srt_last_error()
     torch.cuda.synchronize()
     assert bool((maps64 == 7.0).all())
+
+
+def test_adjoint_in_ray_chunks(srt, setup, monkeypatch):
+    """Rays beyond one chunk's work buffers are processed chunk after chunk: a context limited to
+    128-ray chunks (6 chunks of the 768 rays, and 3 of a 300-entry pixel list with invalid entries, the
+    last chunk partial) gives the maps bit for bit and the gradients to the atomics' summation order."""
+    import torch
+    print("[gpu] adjoint in ray chunks", flush=True)
+    tr, phi, shift, target = setup
+    monkeypatch.setenv("SRT_ADJ_CHUNK_RAYS", "128")
+    small = srt.Tracer(0)
+    monkeypatch.delenv("SRT_ADJ_CHUNK_RAYS")
+    dev = torch.device("cuda:0")
+    for t in (tr, small):
+        t.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
+    maps, _, _ = tr.trace_adjoint(NSIDE, STEPS)
+    _, g = loss_and_grad(maps, target)
+    rng = np.random.default_rng(46)
+    pix = rng.choice(12 * NSIDE * NSIDE, 300, replace=False).astype(np.int64)
+    pix[[7, 150, 299]] = [-1, 12 * NSIDE * NSIDE, -5]
+    gl = torch.from_numpy(rng.standard_normal((6, pix.size))).to(dev)
+    for kw, gm in ((dict(), g), (dict(pix_list=torch.from_numpy(pix).to(dev), count=pix.size), gl)):
+        a = tr.trace_adjoint(NSIDE, STEPS, gm, checkpoint_every=K, **kw)
+        b = small.trace_adjoint(NSIDE, STEPS, gm, checkpoint_every=K, **kw)
+        ma, mb = a[0].cpu().numpy(), b[0].cpu().numpy()
+        assert np.array_equal(np.isnan(ma), np.isnan(mb))
+        assert np.array_equal(ma[~np.isnan(ma)], mb[~np.isnan(mb)])
+        for x, y in zip(a[1:], b[1:]):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            assert np.abs(x - y).max() <= 1e-12 * np.abs(x).max()
+    small.close()
