@@ -102,42 +102,120 @@ __global__ __launch_bounds__(256) void widen_kernel(const float* __restrict__ sr
         dst[q] = (double)src[q];
 }
 
+// Tile (of rays, or of record columns) taken by workgroup `b` of `nb`.  The dispatcher hands workgroups
+// to the chip's XCDs round-robin (b -> XCD b mod 8 on MI355X), and each XCD
+// has its own L2.  With g = 8 groups, the workgroups an XCD receives trace
+// one contiguous range of tiles, so rays of neighbouring sky patches (which
+// read neighbouring cells), or neighbouring record columns, share an L2;
+// g = 1 is the identity.
+__device__ __forceinline__ int64_t tile_of_block(int64_t b, int64_t nb, int g) {
+    if (g <= 1) return b;
+    const int64_t x = b % g, idx = b / g, per = nb / g, extra = nb % g;
+    return x * per + (x < extra ? x : extra) + idx;
+}
+
 // Bicubic cell records of one field from its padded cube (layout: see
-// srt::record_offset).  Thread q = i + n (j + n p) builds the record of column
-// (i, j), padded plane p: the 4 x 4 taps v(i - 1 + a, j - 1 + b) of that plane
-// are padded entries (i + a, j + b, p); the x and y basis changes
-//     c[pw][qw] = sum_{a,b} M[a][pw] M[b][qw] v_ab,  M = srt::kCrMonomial,
-// run in FP64 and each coefficient is rounded to FP32 once.
-__global__ __launch_bounds__(256) void bicubic_records_kernel(const float* __restrict__ padded, int n, int P,
-                                                              int shift, int nb, float* __restrict__ rec) {
-    const int64_t total = (int64_t)n * n * P;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q % n);
-        const int j = (int)((q / n) % n);
-        const int p = (int)(q / ((int64_t)n * n));
-        double X[4][4];  // X[b][pw] = sum_a M[a][pw] v_ab
+// srt::record_offset).  The record of column (i, j), padded plane p holds the
+// 4 x 4 taps v(i - 1 + a, j - 1 + b) of that plane, which are padded entries
+// (i + a, j + b, p), in the monomial basis
+//     c[pw][qw] = sum_{a,b} M[a][pw] M[b][qw] v_ab,  M = srt::kCrMonomial;
+// both basis changes run in FP64 and each coefficient is rounded to FP32 once.
+//
+// A workgroup of kRecRows wavefronts takes a tile of 64 planes p0 .. p0 + 63,
+// kRecRows rows j0 .. and kRecCols columns i0 ..: wavefront w builds row
+// j0 + w, lane l plane p0 + l, column after column.
+//   1. The tile's taps (64 planes x (kRecRows + 3) rows x (kRecCols + 3)
+//      columns) are loaded into LDS; each thread issues all of its loads
+//      before its first LDS write, so their latencies overlap.
+//   2. Each lane keeps the 4 x 4 taps of its current column in registers and
+//      moves them one column on with 4 LDS reads (the 12 others are reused).
+//   3. A column's 64 records (one per lane) are consecutive in memory (4 KB).
+//      They go through a per-wavefront LDS buffer so that each of the 4
+//      stores writes 1 KB contiguous (lane l: 16 bytes at 16 l); stored
+//      directly, each store instruction would scatter 16 bytes per lane over
+//      the 4 KB (3x slower at 512^3).
+// Tiles run (plane block, column block, row block) from fastest to slowest
+// and are grouped by XCD (tile_of_block): an XCD writes the records of a few
+// columns plane block after plane block, close to their order in memory.
+// Records are written with non-temporal stores (nothing reads them back
+// during the build).  512^3: 2.2 ms per field against 5.5 ms for one thread
+// per record in (i, j, p) order.
+constexpr int kRecCols = 16, kRecRows = 4;
+constexpr int kRecTileCols = kRecCols + 3, kRecTileRows = kRecRows + 3;
+constexpr int kRecPlaneStride = kRecTileRows * kRecTileCols + 1;  // odd: lanes (planes) hit distinct banks
+constexpr int kRecTileFloats = 64 * kRecTileRows * kRecTileCols;
+constexpr int kRecLoadsPerThread = (kRecTileFloats + 64 * kRecRows - 1) / (64 * kRecRows);
+
+__global__ __launch_bounds__(64 * kRecRows) void bicubic_records_kernel(const float* __restrict__ padded, int n,
+                                                                       int P, int shift, int nb,
+                                                                       float* __restrict__ rec) {
+    using f4 = float __attribute__((ext_vector_type(4)));
+    __shared__ float taps[64 * kRecPlaneStride];
+    __shared__ f4 staged[kRecRows][64 * 4];
+    const int npb = (P + 63) / 64, ncb = (n + kRecCols - 1) / kRecCols;
+    const int64_t tile = tile_of_block(blockIdx.x, gridDim.x, 8);
+    const int pb = (int)(tile % npb);
+    const int cb = (int)((tile / npb) % ncb);
+    const int rb = (int)(tile / ((int64_t)npb * ncb));
+    const int p0 = pb * 64, i0 = cb * kRecCols, j0 = rb * kRecRows;
+    {
+        float v[kRecLoadsPerThread];
+#pragma unroll
+        for (int u = 0; u < kRecLoadsPerThread; ++u) {
+            const int t = (int)threadIdx.x + u * 64 * kRecRows;
+            const int c = t % kRecTileCols, r = (t / kRecTileCols) % kRecTileRows, pl = t / (kRecTileCols * kRecTileRows);
+            const int gi = i0 + c, gj = j0 + r, gp = p0 + pl;
+            v[u] = 0.0f;
+            if (t < kRecTileFloats && gi < P && gj < P && gp < P)
+                v[u] = padded[(size_t)gi + (size_t)P * ((size_t)gj + (size_t)P * (size_t)gp)];
+        }
+#pragma unroll
+        for (int u = 0; u < kRecLoadsPerThread; ++u) {
+            const int t = (int)threadIdx.x + u * 64 * kRecRows;
+            const int c = t % kRecTileCols, r = (t / kRecTileCols) % kRecTileRows, pl = t / (kRecTileCols * kRecTileRows);
+            if (t < kRecTileFloats) taps[pl * kRecPlaneStride + r * kRecTileCols + c] = v[u];
+        }
+    }
+    __syncthreads();
+    const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const int j = j0 + w;
+    const int nvalid = min(64, P - p0);  // planes of this block
+    const int i_end = min(n, i0 + kRecCols);
+    const float* lane_taps = taps + l * kRecPlaneStride + w * kRecTileCols;
+    f4* const st = staged[w];
+    srt::Grids G;
+    G.pad = P;
+    G.rec_shift = shift;
+    G.rec_nb = nb;
+    double win[4][4];  // win[b][a] = tap (i + a, j + b) of the lane's plane
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) win[b][a + 1] = (double)lane_taps[b * kRecTileCols + a];
+    for (int i = i0; i < i_end; ++i) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float* row = padded + (size_t)i + (size_t)P * ((size_t)(j + b) + (size_t)P * (size_t)p);
-            double v[4];
+            win[b][0] = win[b][1];
+            win[b][1] = win[b][2];
+            win[b][2] = win[b][3];
+            win[b][3] = (double)lane_taps[b * kRecTileCols + (i - i0) + 3];
+        }
+        double X[4][4];  // X[b][pw] = sum_a M[a][pw] v_ab
 #pragma unroll
-            for (int a = 0; a < 4; ++a) v[a] = (double)row[a];
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int pw = 0; pw < 4; ++pw) {
                 double c = 0.0;
 #pragma unroll
-                for (int a = 0; a < 4; ++a) c += srt::kCrMonomial[a][pw] * v[a];
+                for (int a = 0; a < 4; ++a) c += srt::kCrMonomial[a][pw] * win[b][a];
                 X[b][pw] = c;
             }
-        }
-        srt::Grids G;
-        G.pad = P;
-        G.rec_shift = shift;
-        G.rec_nb = nb;
-        float* dst = rec + srt::record_offset(G, i, j, p);
+        // quarter pw of lane l's record goes to slot 4 l + ((pw + l / 4) mod 4)
+        // of the staging buffer (the rotation spreads the 16-byte writes of
+        // lanes 4 apart over the banks)
 #pragma unroll
         for (int pw = 0; pw < 4; ++pw) {
-            float c4[4];
+            f4 c4;
 #pragma unroll
             for (int qw = 0; qw < 4; ++qw) {
                 double c = 0.0;
@@ -145,8 +223,19 @@ __global__ __launch_bounds__(256) void bicubic_records_kernel(const float* __res
                 for (int b = 0; b < 4; ++b) c += srt::kCrMonomial[b][qw] * X[b][pw];
                 c4[qw] = (float)c;
             }
-            __builtin_memcpy(dst + 4 * pw, c4, sizeof(c4));
+            st[l * 4 + ((pw + (l >> 2)) & 3)] = c4;
         }
+        __builtin_amdgcn_wave_barrier();
+        if (j < n) {
+            f4* const run = reinterpret_cast<f4*>(rec + srt::record_offset(G, i, j, p0));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int r = 16 * k + (l >> 2), q = l & 3;  // record r, quarter q
+                const f4 v = st[r * 4 + ((q + (r >> 2)) & 3)];
+                if (r < nvalid) __builtin_nontemporal_store(v, run + 4 * r + q);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -167,17 +256,6 @@ struct TraceArgs {
     int64_t lane_base;          // first lane of this launch (a trace is split into launches of
                                 // at most srt_context::max_launch_rays lanes)
 };
-
-// Ray tile traced by workgroup `b` of `nb`.  The dispatcher hands workgroups
-// to the chip's XCDs round-robin (b -> XCD b mod 8 on MI355X), and each XCD
-// has its own L2.  With g = 8 groups, the workgroups an XCD receives trace
-// one contiguous range of tiles, so rays of neighbouring sky patches (which
-// read neighbouring cells) share an L2; g = 1 is the identity.
-__device__ __forceinline__ int64_t tile_of_block(int64_t b, int64_t nb, int g) {
-    if (g <= 1) return b;
-    const int64_t x = b % g, idx = b / g, per = nb / g, extra = nb % g;
-    return x * per + (x < extra ? x : extra) + idx;
-}
 
 // Traces one ray (RING pixel `pix`) and writes its maps to slot `slot` of
 // the map-major output (and the final state if `state` is not null).
@@ -1031,11 +1109,12 @@ int build_records(srt_context* ctx) {
     ctx->rec_field_floats = field;
     ++ctx->generation;
     const size_t pvox = (size_t)P * P * P;
-    const int64_t total = (int64_t)n * n * P;
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
+    // tiles: 64 planes x kRecCols columns x kRecRows rows (at most 33 x 128 x 512
+    // for the largest grid edge)
+    const int64_t tiles = (int64_t)((P + 63) / 64) * ((n + kRecCols - 1) / kRecCols) * ((n + kRecRows - 1) / kRecRows);
     for (int f = 0; f < 4; ++f)
-        hipLaunchKernelGGL(bicubic_records_kernel, dim3(blocks), dim3(256), 0, 0, ctx->grids + (size_t)f * pvox, n, P,
-                           s, nb, ctx->rec + (size_t)f * field);
+        hipLaunchKernelGGL(bicubic_records_kernel, dim3((unsigned)tiles), dim3(64 * kRecRows), 0, 0,
+                           ctx->grids + (size_t)f * pvox, n, P, s, nb, ctx->rec + (size_t)f * field);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
