@@ -764,9 +764,8 @@ __shared__ double adj_vec[4][19][kAdjRevBlock];
 
 // dL/ds of one stage given dL/dk (adj_vec[kKbar]), into adj_vec[kSbar], from
 // the stage record at `rq` (component c at rq[c * C]); the record's per-field
-// value, gradient and Hessian slots receive their adjoints.  Not inlined: a
-// call boundary keeps the register allocation of the sweep local.
-__device__ __noinline__ void stage_vjp(double* __restrict__ rq_in, int64_t C) {
+// value, gradient and Hessian slots receive their adjoints.
+__device__ __forceinline__ void stage_vjp(double* __restrict__ rq_in, int64_t C) {
     const int l = threadIdx.x;
     const srt::adj::gptr<double> rq = (srt::adj::gptr<double>)rq_in;
     double sv[19], kbar[19], sbar[19], q[40];
@@ -795,30 +794,30 @@ __global__ __launch_bounds__(kAdjRevBlock) void adj_reverse_kernel(AdjArgs A) {
     const int64_t C = A.C;
     double(*const vec)[19][kAdjRevBlock] = adj_vec;
     for (int j = 0; j < 19; ++j) vec[kLam][j][t] = A.lam[j * C + l];
+    const int64_t stage_stride = (int64_t)kAdjRec * C;
     for (int st = A.seg_steps - 1; st >= 0; --st) {
         double* const rq = A.rec + (int64_t)(st * 4) * kAdjRec * C + l;
-        const int64_t stage_stride = (int64_t)kAdjRec * C;
         for (int j = 0; j < 19; ++j) {
             vec[kYbar][j][t] = vec[kLam][j][t];
             vec[kKbar][j][t] = vec[kLam][j][t] * (1.0 / 6.0);
         }
-        stage_vjp(rq + 3 * stage_stride, C);  // s4 = y + k3
-        for (int j = 0; j < 19; ++j) {
-            vec[kYbar][j][t] += vec[kSbar][j][t];
-            vec[kKbar][j][t] = vec[kSbar][j][t] + vec[kLam][j][t] * (1.0 / 3.0);
+        // stages 4, 3, 2, 1 (stage state s4 = y + k3, s3 = y + k2 / 2,
+        // s2 = y + k1 / 2, s1 = y): one copy of the stage VJP in a loop, so
+        // its registers are allocated once and no call saves them
+#pragma nounroll
+        for (int stage = 3; stage >= 0; --stage) {
+            stage_vjp(rq + stage * stage_stride, C);
+            if (stage > 0) {
+                const double c_s = stage == 3 ? 1.0 : 0.5;               // ds/dk of the earlier stage
+                const double c_lam = stage == 1 ? 1.0 / 6.0 : 1.0 / 3.0;  // RK4 weight of that k
+                for (int j = 0; j < 19; ++j) {
+                    vec[kYbar][j][t] += vec[kSbar][j][t];
+                    vec[kKbar][j][t] = c_s * vec[kSbar][j][t] + vec[kLam][j][t] * c_lam;
+                }
+            } else {
+                for (int j = 0; j < 19; ++j) vec[kLam][j][t] = vec[kYbar][j][t] + vec[kSbar][j][t];
+            }
         }
-        stage_vjp(rq + 2 * stage_stride, C);  // s3 = y + k2 / 2
-        for (int j = 0; j < 19; ++j) {
-            vec[kYbar][j][t] += vec[kSbar][j][t];
-            vec[kKbar][j][t] = 0.5 * vec[kSbar][j][t] + vec[kLam][j][t] * (1.0 / 3.0);
-        }
-        stage_vjp(rq + 1 * stage_stride, C);  // s2 = y + k1 / 2
-        for (int j = 0; j < 19; ++j) {
-            vec[kYbar][j][t] += vec[kSbar][j][t];
-            vec[kKbar][j][t] = 0.5 * vec[kSbar][j][t] + vec[kLam][j][t] * (1.0 / 6.0);
-        }
-        stage_vjp(rq, C);  // s1 = y
-        for (int j = 0; j < 19; ++j) vec[kLam][j][t] = vec[kYbar][j][t] + vec[kSbar][j][t];
     }
     for (int j = 0; j < 19; ++j) A.lam[j * C + l] = vec[kLam][j][t];
 }
