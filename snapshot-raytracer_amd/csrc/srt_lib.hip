@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/srt.h"
@@ -93,6 +94,68 @@ __global__ __launch_bounds__(256) void pad_grid_kernel(const float* __restrict__
         float v = src[s * stride + comp];
         if (quantise) v = f16_bits_to_f32(f32_to_f16_bits(v * 134217728.0f)) / 134217728.0f;
         dst[q] = v;
+    }
+}
+
+// Elements [e0, e0 + count) of a host field, staged at src[0 .. count), into
+// the interior of its padded cubes: element e is component e mod ncomp of
+// voxel v = e / ncomp = x + n (y + n z), stored at padded (x + 1, y + 1, z + 1)
+// of cube `comp` (cubes pvox floats apart), with the same optional binary16
+// quantisation as pad_grid_kernel.
+__global__ __launch_bounds__(256) void place_kernel(const float* __restrict__ src, int64_t e0, int64_t count,
+                                                    int ncomp, int n, int P, int quantise, float* __restrict__ cube0,
+                                                    int64_t pvox) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = e0 + q, v = e / ncomp;
+        const int comp = (int)(e - v * ncomp);
+        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+        float val = src[q];
+        if (quantise) val = f16_bits_to_f32(f32_to_f16_bits(val * 134217728.0f)) / 134217728.0f;
+        cube0[comp * pvox + (int64_t)(x + 1) + (int64_t)P * ((int64_t)(y + 1) + (int64_t)P * (z + 1))] = val;
+    }
+}
+
+// The periodic halo of `ncubes` padded cubes (pvox floats apart) from their
+// interiors: padded cell (i, j, k) with a coordinate in {0, n + 1, n + 2}
+// takes interior cell (w(i), w(j), w(k)), w(i) = (i + n - 1) mod n + 1, the
+// value pad_grid_kernel gives it.  Halo cells are enumerated directly: the
+// planes k in {0, n + 1, n + 2} (3 P^2 cells), then for interior k the rows
+// j in {0, n + 1, n + 2} (3 P n), then for interior j and k the cells
+// i in {0, n + 1, n + 2} (3 n^2); every source is an interior cell, so the
+// order of the writes does not matter.
+__global__ __launch_bounds__(256) void halo_kernel(float* __restrict__ cube0, int ncubes, int n, int P,
+                                                   int64_t pvox) {
+    const int64_t a = 3 * (int64_t)P * P, b = a + 3 * (int64_t)P * n, per_cube = b + 3 * (int64_t)n * n;
+    const int64_t total = per_cube * ncubes;
+    auto halo = [n](int h) { return h == 0 ? 0 : n + h; };  // h in {0, 1, 2}
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(q / per_cube);
+        int64_t r = q - (int64_t)c * per_cube;
+        int i, j, k;
+        if (r < a) {
+            k = halo((int)(r / ((int64_t)P * P)));
+            r %= (int64_t)P * P;
+            j = (int)(r / P);
+            i = (int)(r % P);
+        } else if (r < b) {
+            r -= a;
+            j = halo((int)(r / ((int64_t)P * n)));
+            r %= (int64_t)P * n;
+            k = 1 + (int)(r / P);
+            i = (int)(r % P);
+        } else {
+            r -= b;
+            i = halo((int)(r / ((int64_t)n * n)));
+            r %= (int64_t)n * n;
+            k = 1 + (int)(r / n);
+            j = 1 + (int)(r % n);
+        }
+        const int si = (i + n - 1) % n + 1, sj = (j + n - 1) % n + 1, sk = (k + n - 1) % n + 1;
+        float* const cube = cube0 + c * pvox;
+        cube[(int64_t)i + (int64_t)P * ((int64_t)j + (int64_t)P * k)] =
+            cube[(int64_t)si + (int64_t)P * ((int64_t)sj + (int64_t)P * sk)];
     }
 }
 
@@ -992,6 +1055,17 @@ void* scratch_address(int device) {
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI ----
+// Host transfers go through kXferRing pinned chunks of kXferChunkBytes, each
+// filled (or emptied) by kXferThreads host threads while the DMA engine moves
+// the previous one: 55 GB/s host -> device on the MI355X boxes against
+// 26-28 GB/s for hipMemcpy from pageable memory (2 GiB: 39 against 76-81 ms);
+// one thread copies too slowly to keep up (28 GB/s).  The chunk size weighs
+// the first upload of a context, which pins the ring (512^3 from host: 53 ms
+// with 16 MiB chunks, 56 with 32, 65 with 64), against later ones (43, 41,
+// 41 ms): a one-snapshot process, the reference's use, uploads once.
+constexpr int kXferRing = 3, kXferThreads = 4;
+constexpr size_t kXferChunkBytes = size_t(16) << 20;
+
 struct srt_context {
     int device = 0;
     int n = 0;
@@ -1002,8 +1076,19 @@ struct srt_context {
     // adjoint work buffers, grown on demand and kept for later calls
     double* adj_buf = nullptr;
     size_t adj_bytes = 0;
-    // FP64 copies of the padded grids, built on the first FP64 trace
+    // FP64 copies of the padded grids, built on the first FP64 trace after an
+    // upload (the buffer is kept across uploads of the same grid edge)
     double* grids64 = nullptr;
+    bool grids64_valid = false;
+    // transfers between pageable host buffers and the device (srt_upload from
+    // host memory, srt_trace_host): a ring of pinned host buffers and device
+    // buffers of xfer_chunk bytes each, one stream and an event per slot,
+    // created by the first such call and kept
+    hipStream_t xfer = nullptr;
+    size_t xfer_chunk = 0;  // kXferChunkBytes, or less under a per-allocation limit
+    char* xfer_host[kXferRing] = {};
+    float* xfer_dev[kXferRing] = {};
+    hipEvent_t xfer_ev[kXferRing] = {};
     // bicubic cell records of the four fields (SRT_LAYOUT_BICUBIC)
     int layout = SRT_LAYOUT_TAPS;
     int brick_shift = 4;  // 16 x 16 column bricks
@@ -1059,13 +1144,14 @@ hipError_t ctx_malloc(const srt_context* ctx, T** p, size_t bytes) {
 
 // FP64 copies of the padded grids (the FP64 kernel and the adjoint read them)
 int ensure_grids64(srt_context* ctx, hipStream_t st) {
-    if (ctx->grids64) return SRT_OK;
+    if (ctx->grids64 && ctx->grids64_valid) return SRT_OK;
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
-    SRT_HIP(ctx_malloc(ctx, &ctx->grids64, 4 * pvox * sizeof(double)));
+    if (!ctx->grids64) SRT_HIP(ctx_malloc(ctx, &ctx->grids64, 4 * pvox * sizeof(double)));
     ++ctx->generation;
     const int blocks = (int)std::min<int64_t>(((int64_t)(4 * pvox) + 255) / 256, 65536);
     hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, st, ctx->grids, ctx->grids64, (int64_t)(4 * pvox));
     SRT_HIP(hipGetLastError());
+    ctx->grids64_valid = true;
     return SRT_OK;
 }
 
@@ -1073,6 +1159,105 @@ void set_grids64(const srt_context* ctx, srt::Grids& G) {
     const size_t pvox = (size_t)ctx->pad * ctx->pad * ctx->pad;
     G.phi64 = ctx->grids64;
     for (int a = 0; a < 3; ++a) G.b64[a] = ctx->grids64 ? ctx->grids64 + (size_t)(1 + a) * pvox : nullptr;
+}
+
+// memcpy of `bytes` bytes split over kXferThreads host threads
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    const size_t per = (bytes + kXferThreads - 1) / kXferThreads;
+    auto part = [&](int t) {
+        const size_t a = (size_t)t * per;
+        if (a < bytes) std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, std::min(per, bytes - a));
+    };
+    std::thread th[kXferThreads - 1];
+    for (int t = 1; t < kXferThreads; ++t) th[t - 1] = std::thread(part, t);
+    part(0);
+    for (auto& x : th) x.join();
+}
+
+void free_xfer(srt_context* ctx);
+
+// The transfer ring (see kXferRing), created on first use; all of it or none.
+int ensure_xfer(srt_context* ctx) {
+    if (ctx->xfer) return SRT_OK;
+    ctx->xfer_chunk = kXferChunkBytes;
+    if (ctx->max_alloc && ctx->max_alloc < kXferChunkBytes)
+        ctx->xfer_chunk = std::max<size_t>(4096, ctx->max_alloc / 4096 * 4096);
+    hipError_t e = hipStreamCreateWithFlags(&ctx->xfer, hipStreamNonBlocking);
+    for (int k = 0; k < kXferRing && e == hipSuccess; ++k) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&ctx->xfer_host[k]), ctx->xfer_chunk, hipHostMallocDefault);
+        if (e == hipSuccess) e = ctx_malloc(ctx, &ctx->xfer_dev[k], ctx->xfer_chunk);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->xfer_ev[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        free_xfer(ctx);
+        return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
+                    std::string("host transfer buffers: ") + hipGetErrorName(e));
+    }
+    return SRT_OK;
+}
+
+void free_xfer(srt_context* ctx) {
+    for (int k = 0; k < kXferRing; ++k) {
+        if (ctx->xfer_host[k]) (void)hipHostFree(ctx->xfer_host[k]);
+        if (ctx->xfer_dev[k]) (void)hipFree(ctx->xfer_dev[k]);
+        if (ctx->xfer_ev[k]) (void)hipEventDestroy(ctx->xfer_ev[k]);
+        ctx->xfer_host[k] = nullptr;
+        ctx->xfer_dev[k] = nullptr;
+        ctx->xfer_ev[k] = nullptr;
+    }
+    if (ctx->xfer) (void)hipStreamDestroy(ctx->xfer);
+    ctx->xfer = nullptr;
+}
+
+// One host field of `elems` floats (ncomp components per voxel) into the
+// interior of its padded cubes (place_kernel), chunk after chunk through the
+// transfer ring, on the transfer stream.  Returns once every chunk has been
+// copied out of the host buffer (the placement may still run).
+int upload_host_field(srt_context* ctx, const float* host, int64_t elems, int ncomp, int quantise, int n, int P,
+                      float* cube0, int& slot) {
+    const int64_t pvox = (int64_t)P * P * P, chunk = (int64_t)(ctx->xfer_chunk / sizeof(float));
+    for (int64_t e0 = 0; e0 < elems; e0 += chunk) {
+        const int k = slot;
+        slot = (slot + 1) % kXferRing;
+        const int64_t len = std::min(chunk, elems - e0);
+        SRT_HIP(hipEventSynchronize(ctx->xfer_ev[k]));  // the slot's previous chunk has been placed
+        parallel_copy(ctx->xfer_host[k], host + e0, (size_t)len * sizeof(float));
+        SRT_HIP(hipMemcpyAsync(ctx->xfer_dev[k], ctx->xfer_host[k], (size_t)len * sizeof(float),
+                               hipMemcpyHostToDevice, ctx->xfer));
+        const int blocks = (int)std::min<int64_t>((len + 255) / 256, 16384);
+        hipLaunchKernelGGL(place_kernel, dim3(blocks), dim3(256), 0, ctx->xfer, ctx->xfer_dev[k], e0, len, ncomp, n,
+                           P, quantise, cube0, pvox);
+        SRT_HIP(hipGetLastError());
+        SRT_HIP(hipEventRecord(ctx->xfer_ev[k], ctx->xfer));
+    }
+    return SRT_OK;
+}
+
+// `bytes` bytes of device memory into a pageable host buffer through the
+// transfer ring: up to kXferRing chunks in flight, each copied out by
+// kXferThreads host threads once it has arrived.  The caller has finished the
+// work that writes `dev`.
+int download_to_host(srt_context* ctx, void* host, const void* dev, size_t bytes) {
+    const int rc = ensure_xfer(ctx);
+    if (rc != SRT_OK) return rc;
+    const size_t chunk = ctx->xfer_chunk, nchunks = (bytes + chunk - 1) / chunk;
+    auto issue = [&](size_t c) -> hipError_t {
+        const size_t off = c * chunk, len = std::min(chunk, bytes - off);
+        const int k = (int)(c % kXferRing);
+        hipError_t e = hipMemcpyAsync(ctx->xfer_host[k], static_cast<const char*>(dev) + off, len,
+                                      hipMemcpyDeviceToHost, ctx->xfer);
+        if (e == hipSuccess) e = hipEventRecord(ctx->xfer_ev[k], ctx->xfer);
+        return e;
+    };
+    for (size_t c = 0; c < std::min<size_t>(nchunks, kXferRing); ++c) SRT_HIP(issue(c));
+    for (size_t c = 0; c < nchunks; ++c) {
+        const int k = (int)(c % kXferRing);
+        const size_t off = c * chunk, len = std::min(chunk, bytes - off);
+        SRT_HIP(hipEventSynchronize(ctx->xfer_ev[k]));
+        parallel_copy(static_cast<char*>(host) + off, ctx->xfer_host[k], len);
+        if (c + kXferRing < nchunks) SRT_HIP(issue(c + kXferRing));
+    }
+    return SRT_OK;
 }
 
 int brick_columns(int n, int shift) {
@@ -1225,6 +1410,7 @@ int srt_destroy(srt_context* ctx) {
     if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
     if (ctx->grids64) (void)hipFree(ctx->grids64);
     free_records(ctx);
+    free_xfer(ctx);
     if (ctx->order_buf) (void)hipFree(ctx->order_buf);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
@@ -1255,52 +1441,54 @@ int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64
     const int n = (int)nx, P = n + 3;
     const size_t nvox = (size_t)n * n * n, pvox = (size_t)P * P * P;
     ++ctx->generation;
-    if (ctx->grids) {
+    // Buffers of the same grid edge are kept (kernels still reading them are
+    // waited for); others are freed before the new grids are allocated.
+    SRT_HIP(hipDeviceSynchronize());
+    if (ctx->grids && ctx->n != n) {
         SRT_HIP(hipFree(ctx->grids));
         ctx->grids = nullptr;
-        ctx->n = ctx->pad = 0;
-    }
-    if (ctx->grids64) {
-        SRT_HIP(hipFree(ctx->grids64));
+        if (ctx->grids64) SRT_HIP(hipFree(ctx->grids64));
         ctx->grids64 = nullptr;
+        free_records(ctx);
     }
-    free_records(ctx);
-    float* grids = nullptr;
-    SRT_HIP(ctx_malloc(ctx, &grids, 4 * pvox * sizeof(float)));
-    const float* dphi = phi;
-    const float* dshift = shift_xyz;
-    float* staging = nullptr;
-    if (!inputs_on_device) {
-        hipError_t e = ctx_malloc(ctx, &staging, 4 * nvox * sizeof(float));
-        if (e != hipSuccess) {
-            (void)hipFree(grids);
-            return fail(e == hipErrorOutOfMemory ? SRT_ERR_OUT_OF_MEMORY : SRT_ERR_DEVICE,
-                        std::string("srt_upload: staging hipMalloc: ") + hipGetErrorName(e));
-        }
-        e = hipMemcpy(staging, phi, nvox * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(staging + nvox, shift_xyz, 3 * nvox * sizeof(float), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(staging);
-            (void)hipFree(grids);
-            return fail(SRT_ERR_DEVICE, std::string("srt_upload: hipMemcpy: ") + hipGetErrorName(e));
-        }
-        dphi = staging;
-        dshift = staging + nvox;
+    if (ctx->layout != SRT_LAYOUT_BICUBIC) free_records(ctx);  // kept only to be rebuilt in place below
+    ctx->grids64_valid = false;
+    ctx->n = ctx->pad = 0;  // no snapshot until this upload completes
+    if (!ctx->grids) SRT_HIP(ctx_malloc(ctx, &ctx->grids, 4 * pvox * sizeof(float)));
+    float* const grids = ctx->grids;
+    const int quantise = shift_format == SRT_SHIFT_F16_SCALED ? 1 : 0;
+    auto abandon = [&](int code, const std::string& msg) {
+        // the context holds no snapshot after a failed upload
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ctx->grids);
+        ctx->grids = nullptr;
+        if (ctx->grids64) (void)hipFree(ctx->grids64);
+        ctx->grids64 = nullptr;
+        free_records(ctx);
+        return fail(code, msg);
+    };
+    if (inputs_on_device) {
+        const int blocks = (int)std::min<int64_t>(((int64_t)pvox + 255) / 256, 65536);
+        hipLaunchKernelGGL(pad_grid_kernel, dim3(blocks), dim3(256), 0, 0, phi, (int64_t)1, 0, n, P, 0, grids);
+        for (int a = 0; a < 3; ++a)
+            hipLaunchKernelGGL(pad_grid_kernel, dim3(blocks), dim3(256), 0, 0, shift_xyz, (int64_t)3, a, n, P,
+                               quantise, grids + (size_t)(1 + a) * pvox);
+    } else {
+        // host buffers: chunks through the transfer ring into the cubes'
+        // interiors, then the periodic halo
+        int rc = ensure_xfer(ctx);
+        int slot = 0;
+        if (rc == SRT_OK) rc = upload_host_field(ctx, phi, (int64_t)nvox, 1, 0, n, P, grids, slot);
+        if (rc == SRT_OK)
+            rc = upload_host_field(ctx, shift_xyz, 3 * (int64_t)nvox, 3, quantise, n, P, grids + pvox, slot);
+        if (rc != SRT_OK) return abandon(rc, "srt_upload: " + g_last_error);
+        const int64_t halo = 4 * (3 * (int64_t)P * P + 3 * (int64_t)P * n + 3 * (int64_t)n * n);
+        const int blocks = (int)std::min<int64_t>((halo + 255) / 256, 65536);
+        hipLaunchKernelGGL(halo_kernel, dim3(blocks), dim3(256), 0, ctx->xfer, grids, 4, n, P, (int64_t)pvox);
     }
-    const int blocks = (int)std::min<int64_t>(((int64_t)pvox + 255) / 256, 65536);
-    hipLaunchKernelGGL(pad_grid_kernel, dim3(blocks), dim3(256), 0, 0, dphi, (int64_t)1, 0, n, P, 0, grids);
-    for (int a = 0; a < 3; ++a)
-        hipLaunchKernelGGL(pad_grid_kernel, dim3(blocks), dim3(256), 0, 0, dshift, (int64_t)3, a, n, P,
-                           shift_format == SRT_SHIFT_F16_SCALED ? 1 : 0, grids + (size_t)(1 + a) * pvox);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (staging) (void)hipFree(staging);
-    if (e != hipSuccess) {
-        (void)hipFree(grids);
-        return fail(SRT_ERR_DEVICE, std::string("srt_upload: pad kernel: ") + hipGetErrorName(e));
-    }
-    ctx->grids = grids;
+    if (e != hipSuccess) return abandon(SRT_ERR_DEVICE, std::string("srt_upload: ") + hipGetErrorName(e));
     ctx->n = n;
     ctx->pad = P;
     if (ctx->layout == SRT_LAYOUT_BICUBIC) {
@@ -2157,11 +2345,10 @@ int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t c
                    nullptr);
     if (rc == SRT_OK) {
         e = hipDeviceSynchronize();
-        if (e == hipSuccess && c) e = hipMemcpy(out, d_out, 6 * c * sizeof(float), hipMemcpyDeviceToHost);
-        if (e == hipSuccess && final_state && c)
-            e = hipMemcpy(final_state, d_state, 19 * c * sizeof(double), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(SRT_ERR_DEVICE, std::string("srt_trace_host: ") + hipGetErrorName(e));
     }
+    if (rc == SRT_OK && c) rc = download_to_host(ctx, out, d_out, 6 * c * sizeof(float));
+    if (rc == SRT_OK && c && final_state) rc = download_to_host(ctx, final_state, d_state, 19 * c * sizeof(double));
     cleanup();
     return rc;
 }
