@@ -118,10 +118,15 @@ const char* srt_last_error(void);
  *   shift_format: srt_shift_format
  * The grid must be cubic (nx == ny == nz, 4 <= nx <= 2048); the reference
  * silently assumes equal dimensions for both files (.cu:645, 676, 752-772).
- * Replaces any previous snapshot of the context.  If the context's layout is
- * SRT_LAYOUT_BICUBIC and the records do not fit, the snapshot is still
- * uploaded, the context falls back to SRT_LAYOUT_TAPS and the call returns
- * SRT_ERR_OUT_OF_MEMORY. */
+ * Replaces any previous snapshot of the context (a snapshot of the same edge
+ * reuses its device buffers).  Host inputs go through the context's pinned
+ * transfer ring (3 x 16 MiB of pinned host memory and as much device memory,
+ * allocated by the first host upload or srt_trace_host and kept until
+ * srt_destroy), copied into it by 4 host threads; the caller's buffers need
+ * not be pinned.  If the context's layout is SRT_LAYOUT_BICUBIC and the
+ * records do not fit, the snapshot is still uploaded, the context falls back
+ * to SRT_LAYOUT_TAPS and the call returns SRT_ERR_OUT_OF_MEMORY.  A failed
+ * upload of any other kind leaves the context without a snapshot. */
 int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64_t nx, int64_t ny, int64_t nz,
                int inputs_on_device, int shift_format);
 
@@ -188,7 +193,8 @@ int srt_trace_noalloc(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
 
 /* Synchronous host-memory variant of srt_trace for callers without device
  * buffers (the CLI): pix_list, out (6 * count floats) and final_state are host
- * pointers; pix_list and final_state may be null. */
+ * pointers; pix_list and final_state may be null.  The outputs come back
+ * through the context's transfer ring (see srt_upload). */
 int srt_trace_host(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_t count, const int64_t* pix_list,
                    int n_steps, const double* observer, int precision, int ray_order, float* out,
                    double* final_state);
