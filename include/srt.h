@@ -125,8 +125,9 @@ const char* srt_last_error(void);
  * srt_destroy), copied into it by 4 host threads; the caller's buffers need
  * not be pinned.  If the context's layout is SRT_LAYOUT_BICUBIC and the
  * records do not fit, the snapshot is still uploaded, the context falls back
- * to SRT_LAYOUT_TAPS and the call returns SRT_ERR_OUT_OF_MEMORY.  A failed
- * upload of any other kind leaves the context without a snapshot. */
+ * to SRT_LAYOUT_TAPS and the call returns SRT_ERR_OUT_OF_MEMORY.  Rejected
+ * arguments leave the previous snapshot in place; any other failure leaves
+ * the context without a snapshot. */
 int srt_upload(srt_context* ctx, const float* phi, const float* shift_xyz, int64_t nx, int64_t ny, int64_t nz,
                int inputs_on_device, int shift_format);
 
