@@ -1161,17 +1161,31 @@ void set_grids64(const srt_context* ctx, srt::Grids& G) {
     for (int a = 0; a < 3; ++a) G.b64[a] = ctx->grids64 ? ctx->grids64 + (size_t)(1 + a) * pvox : nullptr;
 }
 
-// memcpy of `bytes` bytes split over kXferThreads host threads
+// memcpy of `bytes` bytes split over kXferThreads host threads (parts whose
+// thread cannot be started are copied by the calling thread: no exception
+// leaves the C ABI)
 void parallel_copy(void* dst, const void* src, size_t bytes) {
     const size_t per = (bytes + kXferThreads - 1) / kXferThreads;
-    auto part = [&](int t) {
+    auto part = [dst, src, bytes, per](int t) {
         const size_t a = (size_t)t * per;
         if (a < bytes) std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, std::min(per, bytes - a));
     };
     std::thread th[kXferThreads - 1];
-    for (int t = 1; t < kXferThreads; ++t) th[t - 1] = std::thread(part, t);
+    bool started[kXferThreads - 1] = {};
+    for (int t = 1; t < kXferThreads; ++t) {
+        try {
+            th[t - 1] = std::thread(part, t);
+            started[t - 1] = true;
+        } catch (...) {
+        }
+    }
     part(0);
-    for (auto& x : th) x.join();
+    for (int t = 1; t < kXferThreads; ++t) {
+        if (started[t - 1])
+            th[t - 1].join();
+        else
+            part(t);
+    }
 }
 
 void free_xfer(srt_context* ctx);
