@@ -118,3 +118,22 @@ def test_host_outputs_in_chunks(srt, snaps):
     assert np.array_equal(bits(maps_h), bits(out.view(6, count).cpu().numpy()))
     assert np.array_equal(bits(state_h), bits(state.view(count, 19).cpu().numpy()))
     tr.close()
+
+
+@pytest.mark.parametrize("n", [4, 5, 33])
+def test_small_grids_host_upload(srt, fields, n):
+    """Edges where the halo is most of the padded cube (n = 4: 343 padded
+    cells, 279 of them halo): host and device uploads trace alike."""
+    print(f"[gpu] host vs device upload, {n}^3", flush=True)
+    phi, shift = fields.snapshot(n, 50 + n)
+    phi, shift = phi.numpy(), fields.interleave_shift(shift).numpy()
+    for fmt in (srt.SHIFT_F32, srt.SHIFT_F16_SCALED):
+        a = srt.Tracer(0)
+        a.set_layout(srt.LAYOUT_BICUBIC, 16)
+        a.upload(phi, shift, fmt)
+        b = from_device(srt, phi, shift, fmt, layout=(srt.LAYOUT_BICUBIC, 16))
+        ma, mb = traced(srt, a, ("mixed", "fp64", "ref")), traced(srt, b, ("mixed", "fp64", "ref"))
+        for p in ma:
+            assert np.array_equal(bits(ma[p]), bits(mb[p])), (p, fmt)
+        a.close()
+        b.close()
