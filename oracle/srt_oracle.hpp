@@ -27,10 +27,14 @@
 // A third instantiation with an operation-counting scalar type counts flops.
 //
 // Pins (tests/): every function here is checked against closed forms, brute
-// force or first-order physics, except two terms of rhs, which are checked
-// only against an independent tensor-form restatement of the same equations
-// (test_oracle_physics.py): L (nLaplaceB, entering only D'') and the
-// antisymmetric part of E1, E2.  For those two terms parity is unpinned.
+// force or first-order physics.  Every term of rhs is pinned by linearised
+// general relativity (test_oracle_shift_gr.py: geodesic, Jacobi map and
+// parallel-transported frame of the metric -(1+2 phi) dt^2 - 2 B.dx dt +
+// (1-2 phi) dx^2 for a divergence-free shift and for a potential) and, for
+// the shift terms, by a pure-gradient shift (test_oracle_shift_gauge.py).
+// The REF form's float rounding points are checked only to tolerances: the
+// reference is built with --use_fast_math, so its float results are not
+// reproducible bit for bit on any host.
 //
 // Periodic wrap: the reference forms (i + o + 20 n) % n (.cu:241-256), valid
 // while a ray stays within 20 boxes on the negative side.  This restatement

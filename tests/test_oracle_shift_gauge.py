@@ -20,9 +20,9 @@ order), the right-hand side's shift terms must cancel in a known way:
   symmetric shift gradient E1 = 1/2 sum_a [e1_a (n . grad B_a) + n_a (e1 . grad B_a)]
   equals N . e1, so the screen vectors stay orthogonal to n (.cu:383-387,
   473-479).  This pins the symmetric part of E1, E2 (their factor 1/2 and
-  the sum of the two terms); the antisymmetric part, and L (nLaplaceB, which
-  enters only D''), are pinned by the tensor-form restatement in
-  test_oracle_physics.py alone.
+  the sum of the two terms); their antisymmetric part, L (nLaplaceB, which
+  enters only D'') and T beyond its gradient-shift cancellation are pinned by
+  a divergence-free shift in test_oracle_shift_gr.py.
 
 The shift samples are exact: B_a at grid node g is d_a chi(g + e_a / 2), the
 point the oracle samples when it evaluates B_a at x - e_a / 2.  chi is a sum
