@@ -97,12 +97,13 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0,
                    help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU this process "
                         "may run on)")
-    p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch", "layouts"],
+    p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch", "layouts", "cli"],
                    help="headline: 512^3 / Nside 1024 / 256 steps (the JSON line the driver reads); "
                         "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
                         "Nside 64 / 64 steps gradient with finite-difference check; batch: 48 uneven "
                         "instances in one plan launch (persistent and dispatch schedules); layouts: the headline workload's mixed "
-                        "kernel on the padded cubes and on bicubic records of several brick edges")
+                        "kernel on the padded cubes and on bicubic records of several brick edges; cli: the "
+                        "command-line drop-in end to end on the headline workload (HDF5 files in, FITS maps out)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -292,6 +293,56 @@ def workload_adjoint(args, pkg, fields, dev):
                                                    "shift_z": worst[3]}}))
 
 
+def workload_cli(args, pkg, fields, dev):
+    """The drop-in end to end: the 512^3 snapshot (seed 2) written as the two HDF5 files the reference reads,
+    then the command-line tool at Nside 1024 / 256 steps, full sky (HDF5 in, six FITS maps out), timed as a
+    process and by phase; its dA map is checked bit for bit against a library trace of the same snapshot."""
+    import re
+    import subprocess
+    import tempfile
+    import numpy as np
+    from snapshot_raytracer_amd import snapshot_io
+    n, nside, n_steps = args.grid, args.nside, args.n_steps
+    cli = os.path.join(ROOT, "snapshot-raytracer_amd", "bin", "snapshot-raytracer")
+    if not os.path.exists(cli) or not snapshot_io.hdf5_available():
+        print(json.dumps({"workload": "cli", "skipped": "CLI not built or HDF5 library absent"}))
+        return
+    phi, shift_il, _ = synth(fields, n, args.seed, dev)
+    phi_h, shift_h = phi.cpu().numpy(), shift_il.cpu().numpy()
+    del phi, shift_il
+    with tempfile.TemporaryDirectory(prefix="srt_cli_") as d:
+        t0 = time.perf_counter()
+        snapshot_io.write_potential(os.path.join(d, "phi.h5"), phi_h)
+        snapshot_io.write_shift(os.path.join(d, "B.h5"), shift_h)
+        write_s = time.perf_counter() - t0
+        print(f"[bench] cli: HDF5 inputs written in {write_s:.2f} s", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        r = subprocess.run([cli, "phi.h5", "B.h5", str(nside), str(n_steps)], cwd=d, capture_output=True,
+                           text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            print(r.stdout[-2000:], r.stderr[-2000:], file=sys.stderr)
+            raise SystemExit(f"cli failed with status {r.returncode}")
+        got, _ = snapshot_io.read_healpix_fits(os.path.join(d, "dAmap_batch0.fits"))
+    timing = re.search(r"Timing: HDF5 read ([0-9.e+-]+) s, device setup ([0-9.e+-]+) s, trace with map download "
+                       r"([0-9.e+-]+) s, FITS write ([0-9.e+-]+) s, total ([0-9.e+-]+) s", r.stdout)
+    kernel = re.search(r"kernel ([0-9.e+-]+) ms", r.stdout)
+    tr = pkg.Tracer(0)
+    tr.set_layout(pkg.LAYOUT_BICUBIC, 16)
+    tr.upload(phi_h, shift_h)
+    ref = tr.trace(nside, n_steps, precision=pkg.PREC_MIXED, order=pkg.ORDER_NESTED)
+    tr.close()
+    rays = 12 * nside * nside
+    phases = dict(zip(["hdf5_read_s", "device_setup_s", "trace_with_map_download_s", "fits_write_s", "total_s"],
+                      (float(x) for x in timing.groups()))) if timing else None
+    print(json.dumps({"workload": "cli", "config": {"grid": n, "nside": nside, "n_steps": n_steps, "rays": rays,
+                                                    "precision": "mixed", "layout": "auto (bicubic records)"},
+                      "process_wall_s": wall, "phases": phases,
+                      "kernel_ms": float(kernel.group(1)) if kernel else None,
+                      "ray_steps_per_s_process": rays * n_steps / wall,
+                      "dA_map_equals_library_trace": bool(np.array_equal(got, ref[0]))}))
+
+
 def workload_batch(args, pkg, fields, dev):
     """48 instances, grids from {128,192,256,384}^3, observers uniform in the box, n_steps in
     [64, N/2], Nside from {128,256,512}, one random RING batch each (rng 7, seeds 1000 + 4 i).
@@ -451,7 +502,7 @@ def main():
         del src, dst
     if args.workload != "headline":
         {"large": workload_large, "adjoint": workload_adjoint, "batch": workload_batch,
-         "layouts": workload_layouts}[args.workload](
+         "layouts": workload_layouts, "cli": workload_cli}[args.workload](
             args, pkg, fields, dev)
         return
     n, nside, n_steps = args.grid, args.nside, args.n_steps

@@ -110,6 +110,10 @@ int main(int argc, char* argv[]) {
 
     std::cout << "Nside: " << nside << std::endl;
     std::cout << "n_steps: " << n_steps << std::endl;
+    // wall time of each phase, printed before the final line
+    using clock = std::chrono::steady_clock;
+    const auto t_begin = clock::now();
+    auto seconds = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
     std::cout << "Loading potential data from HDF5 file " << argv[1] << " ..." << std::endl;
     std::string err;
     std::vector<float> phi;
@@ -142,6 +146,7 @@ int main(int argc, char* argv[]) {
         }
     }
 
+    const auto t_read = clock::now();
     std::cout << "Allocating device memory and copying data to device..." << std::endl;
     srt_context* ctx = nullptr;
     int rc = srt_create((int)device, &ctx);
@@ -176,10 +181,11 @@ int main(int argc, char* argv[]) {
     std::cout << "Launching raytracing kernel (" << prec_s << ", "
               << (lane_order == SRT_ORDER_NESTED ? "nested" : "linear") << " lane order)..." << std::endl;
     std::vector<float> maps((size_t)(6 * count));
-    const auto t0 = std::chrono::steady_clock::now();
+    const auto t0 = clock::now();
     rc = srt_trace_host(ctx, nside, istart, count, nullptr, (int)n_steps, nullptr, precision, order, maps.data(),
                         nullptr);
-    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto t_trace = clock::now();
+    const double sec = seconds(t0, t_trace);
     float kernel_ms = 0.0f;
     if (rc == SRT_OK) rc = srt_last_trace_ms(ctx, &kernel_ms);
     srt_destroy(ctx);
@@ -205,6 +211,10 @@ int main(int argc, char* argv[]) {
             return -1;
         }
     }
+    const auto t_end = clock::now();
+    std::cout << "Timing: HDF5 read " << seconds(t_begin, t_read) << " s, device setup " << seconds(t_read, t0)
+              << " s, trace with map download " << sec << " s, FITS write " << seconds(t_trace, t_end)
+              << " s, total " << seconds(t_begin, t_end) << " s" << std::endl;
     std::cout << "All done!" << std::endl;
     return 0;
 }
