@@ -53,6 +53,26 @@ PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3, "mixed_taps": 157.3, "ref": 157.3}
 KERNELS = {"mixed": ("mixed", "bicubic"), "mixed_taps": ("mixed", "taps"), "fp64": ("fp64", None),
            "ref": ("ref", None)}
 UNPACKED_FP32_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+# HBM bytes of one 128^3 / Nside 64 / 64-step / K = 16 gradient pass: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# summed over its dispatches (profiles/adjoint_kernels_128cube_nside64_kernel_trace.md)
+ADJOINT_HBM_BYTES_PER_PASS = 36.9e9
+# doubles of one stage record of the adjoint (state, then value, gradient, Hessian and third derivatives of
+# the four fields) and of the interpolation adjoints one stage writes (10 per field)
+ADJ_STAGE_RECORD_DOUBLES = 19 + 4 * 20
+ADJ_STAGE_BARS_DOUBLES = 4 * 10
+
+
+def adjoint_pass_bytes(rays, n_steps, K):
+    """Algorithmic device-memory bytes of one gradient pass: every stage record written once (by the forward
+    sweep or a segment's recompute) and read once (reverse sweep); the interpolation adjoints written once
+    (reverse sweep) and read once with the stage positions (scatter, per field); checkpoints written once and
+    read once; dL/dy and the maps."""
+    stages = 4 * n_steps
+    nseg = (n_steps + K - 1) // K
+    per_ray = 8 * (2 * stages * ADJ_STAGE_RECORD_DOUBLES + 2 * stages * ADJ_STAGE_BARS_DOUBLES
+                   + stages * 4 * 3 + 2 * (nseg + 1) * 19 + 2 * nseg * 19 + 12)
+    return rays * per_ray
 
 
 def host_cpu():
@@ -217,8 +237,17 @@ def workload_large(args, pkg, fields, dev):
         print(f"[bench] large: bicubic records not built ({e})", file=sys.stderr, flush=True)
     tr.set_layout(pkg.LAYOUT_TAPS)
     run("taps", both)
+    roof = None
+    if isinstance(res.get("bicubic_brick16"), dict):
+        ms = res["bicubic_brick16"]["nested"]["kernel_ms"]
+        ach = ALGO_FLOPS_PER_RAY_STEP * rays * n_steps / (ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
+                "frac_kernel_time_only": ach / PEAK_TFLOPS["mixed"],
+                "algorithmic_flops_per_ray_step": ALGO_FLOPS_PER_RAY_STEP,
+                "kernel": "trace_kernel<mixed, bicubic records>, 16x16 bricks, nested lanes"}
     print(json.dumps({"workload": "large", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
-                                                      "rays": rays, "precision": "mixed"}, "results": res}))
+                                                      "rays": rays, "precision": "mixed"}, "results": res,
+                      "roofline": roof}))
 
 
 def workload_adjoint(args, pkg, fields, dev):
@@ -284,9 +313,17 @@ def workload_adjoint(args, pkg, fields, dev):
         checks.append((comp, fd, ad))
     scale = {c: max(abs(x[1]) for x in checks if x[0] == c) for c in range(4)}
     worst = {c: max(abs(x[1] - x[2]) for x in checks if x[0] == c) / scale[c] for c in range(4)}
+    algo_bytes = adjoint_pass_bytes(count, n_steps, K)
     print(json.dumps({"workload": "adjoint", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
                                                         "checkpoint_every": K, "rays": count},
                       "adjoint_wall_s": adj_s, "adjoint_kernel_ms": adj_kernel_ms,
+                      "roofline": {"bound": "hbm", "achieved": algo_bytes / (adj_kernel_ms * 1e-3) / 1e9,
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": algo_bytes / (adj_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "algorithmic_bytes_per_pass": algo_bytes,
+                                   "traffic": ADJOINT_HBM_BYTES_PER_PASS,
+                                   "kernel": "the gradient pass (forward sweep, recomputes, reverse sweeps, "
+                                             "scatters): stage records through HBM"},
                       "ray_steps_per_s": count * n_steps / adj_s,
                       "fd_voxels": len(checks),
                       "max_abs_diff_over_max_fd": {"potential": worst[0], "shift_x": worst[1], "shift_y": worst[2],
@@ -444,7 +481,13 @@ def workload_batch(args, pkg, fields, dev):
                       "plan_default_config": {"tile_rays": default_cfg[0], "blocks": default_cfg[1]},
                       "plan_variants_ms": variants,
                       "autotuned": {"schedule": schedule, "tile_rays": tile, "blocks": blocks, "brick": brick},
-                      "plan": res}))
+                      "plan": res,
+                      "roofline": {"bound": "valu", "unit": "TFLOP/s", "peak": PEAK_TFLOPS["mixed"],
+                                   "achieved": ALGO_FLOPS_PER_RAY_STEP * total_rs / (res["direct"]["kernel_ms"] * 1e-3) / 1e12,
+                                   "frac_kernel_time_only": ALGO_FLOPS_PER_RAY_STEP * total_rs
+                                   / (res["direct"]["kernel_ms"] * 1e-3) / 1e12 / PEAK_TFLOPS["mixed"],
+                                   "algorithmic_flops_per_ray_step": ALGO_FLOPS_PER_RAY_STEP,
+                                   "kernel": "the autotuned plan launch (" + schedule + " schedule)"}}))
     plan.close()
 
 
