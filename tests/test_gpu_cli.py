@@ -97,3 +97,39 @@ def test_cli_records_that_do_not_fit(srt, files):
     assert np.array_equal(got, ref[0])
     r = run(d, ["4", "6"], dict(limit, SRT_LAYOUT="bicubic"))
     assert r.returncode == 255
+
+
+def test_cli_headline_scale(srt, fields, tmp_path):
+    """The drop-in at the benchmark's scale: the 512^3 snapshot (seed 2) as the two HDF5 files, the full sky at
+    Nside 1024 with 256 steps, then batch 5 alone.  Every FITS map equals the library's trace of the same
+    snapshot bit for bit (mixed kernel on the bicubic records, which the CLI builds when they fit), and the
+    batch's maps are exactly zero outside its RING range."""
+    from snapshot_raytracer_amd import snapshot_io
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built (no HDF5)")
+    n, nside, steps, batch = 512, 1024, 256, 5
+    print(f"[gpu] cli headline scale: {n}^3 snapshot to HDF5", flush=True)
+    phi, shift = fields.snapshot(n, 2, device="cuda:0")
+    phi_h = phi.cpu().numpy()
+    shift_h = fields.interleave_shift(shift).cpu().numpy()
+    del phi, shift
+    snapshot_io.write_potential(str(tmp_path / "phi.h5"), phi_h)
+    snapshot_io.write_shift(str(tmp_path / "B.h5"), shift_h)
+    r = run(tmp_path, [str(nside), str(steps)])
+    assert r.returncode == 0, r.stderr
+    assert "Timing: HDF5 read" in r.stdout and r.stdout.rstrip().endswith("All done!")
+    r_b = run(tmp_path, [str(nside), str(steps), str(batch)])
+    assert r_b.returncode == 0, r_b.stderr
+    tr = srt.Tracer(0)
+    tr.set_layout(srt.LAYOUT_BICUBIC, 16)
+    tr.upload(phi_h, shift_h)
+    ref = tr.trace(nside, steps, precision=srt.PREC_MIXED, order=srt.ORDER_NESTED)
+    tr.close()
+    lo, cnt = srt.batch_range(nside, batch)
+    for m, name in enumerate(NAMES):
+        got, hdr = snapshot_io.read_healpix_fits(str(tmp_path / f"{name}_batch0.fits"))
+        assert hdr["NSIDE"] == nside and got.size == 12 * nside * nside
+        assert np.array_equal(got, ref[m]), name
+        got_b, _ = snapshot_io.read_healpix_fits(str(tmp_path / f"{name}_batch{batch}.fits"))
+        assert np.all(got_b[:lo] == 0) and np.all(got_b[lo + cnt:] == 0), name
+        assert np.array_equal(got_b[lo:lo + cnt], ref[m][lo:lo + cnt]), name
