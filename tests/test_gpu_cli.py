@@ -133,3 +133,27 @@ def test_cli_headline_scale(srt, fields, tmp_path):
         got_b, _ = snapshot_io.read_healpix_fits(str(tmp_path / f"{name}_batch{batch}.fits"))
         assert np.all(got_b[:lo] == 0) and np.all(got_b[lo + cnt:] == 0), name
         assert np.array_equal(got_b[lo:lo + cnt], ref[m][lo:lo + cnt]), name
+
+
+def test_cli_batches_add_up_to_full_sky(files):
+    """README.md:41-42 of the reference: batch outputs are zero outside their base-pixel range, "so the maps can
+    simply be added together".  The 12 batch runs' FITS maps, summed, equal the full-sky run's bit for bit
+    (each pixel is non-zero in exactly one batch), at a Nside that is not a power of two."""
+    from snapshot_raytracer_amd import snapshot_io
+    d = files[0]
+    nside, steps = 6, 8
+    r = run(d, [str(nside), str(steps)])
+    assert r.returncode == 0, r.stderr
+    full = [snapshot_io.read_healpix_fits(str(d / f"{name}_batch0.fits"))[0].copy() for name in NAMES]
+    total = [np.zeros_like(m) for m in full]
+    for b in range(12):
+        r = run(d, [str(nside), str(steps), str(b)])
+        assert r.returncode == 0, r.stderr
+        for m, name in enumerate(NAMES):
+            got, hdr = snapshot_io.read_healpix_fits(str(d / f"{name}_batch{b}.fits"))
+            assert hdr["NSIDE"] == nside
+            total[m] += got
+    for m, name in enumerate(NAMES):
+        assert np.array_equal(total[m], full[m]), name
+    r = run(d, [str(nside), str(steps), "12"])  # beyond the last batch: the reference's message and status
+    assert r.returncode == 255 and "Batch exceeds total number of pixels" in r.stderr
