@@ -45,7 +45,7 @@ ALGO_FLOPS_PER_RAY_STEP = 12290.5
 # The same count for the reference's own form (three expanded Catmull-Rom
 # polynomials per row and pass, .cu:195-312), for comparison.
 REF_FORM_FLOPS_PER_RAY_STEP = 29298.5
-# MI355X published vector peaks (FP32 counts packed FP32; this build disables
+# MI355X published vector peaks (FP32 counts packed FP32; the library disables
 # packed FP32, so 78.6 TFLOP/s is the FP32 rate of the instructions it issues).
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3, "mixed_taps": 157.3, "ref": 157.3}
 # kernel name -> (precision, device layout it reads); None = the padded cubes
@@ -118,7 +118,7 @@ def parse():
                    help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU this process "
                         "may run on)")
     p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch", "layouts", "cli"],
-                   help="headline: 512^3 / Nside 1024 / 256 steps (the JSON line the driver reads); "
+                   help="headline: 512^3 / Nside 1024 / 256 steps (the benchmark's JSON line); "
                         "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
                         "Nside 64 / 64 steps gradient with finite-difference check; batch: 48 uneven "
                         "instances in one plan launch (persistent and dispatch schedules); layouts: the headline workload's mixed "
@@ -526,7 +526,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the ray tracer shards rays "
-              "across GPUs by RING batches with no exchange, and this build measures the one-GPU share.",
+              "across GPUs by RING batches with no exchange, and the benchmark measures the one-GPU share.",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -38,7 +38,7 @@ typedef enum {
     SRT_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed */
     SRT_ERR_NOT_UPLOADED = 4,     /* srt_trace before srt_upload */
     SRT_ERR_SHAPE = 5,            /* non-cubic or inconsistent grid dimensions */
-    SRT_ERR_UNSUPPORTED = 6       /* valid request this build cannot serve */
+    SRT_ERR_UNSUPPORTED = 6       /* valid request the library cannot serve */
 } srt_status;
 
 /* Arithmetic of the ray kernel. */

@@ -1,7 +1,7 @@
 """ctypes binding of the host oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
-the CPU-baseline leg of ``bench.py``, always as the checker, never as the
+the CPU-baseline leg of ``bench.py``, always as the reference results, never as the
 thing measured for the GPU path or shipped with it.
 
 The oracle restates snapshot-raytracer.cu:195-583 on the host (see

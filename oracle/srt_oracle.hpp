@@ -1,6 +1,6 @@
 // Host restatement of the snapshot ray tracer's per-ray computation.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker: it is compiled into
+// TEST INFRASTRUCTURE ONLY.  This file gives the reference results: it is compiled into
 // oracle/liboracle.so, which only the tests, the smoke check and the CPU
 // baseline leg of the benchmark load.  The product path (the HIP library) never
 // includes or calls it.
