@@ -149,7 +149,9 @@ int srt_get_layout(srt_context* ctx, int* layout, int* brick, int64_t* record_by
  *   observer: 3 doubles in grid units (host pointer); null = (nx, ny, nz) / 2 (.cu:560).
  *   count: rays; the library cannot see the output buffers, so the caller
  *        guarantees they hold count rays (a trace of 2^32 rays or more runs
- *        as several launches).
+ *        as several launches).  A RING range [pix_begin, pix_begin + count)
+ *        must lie within the sky; a pixel list may have any length and repeat
+ *        pixels, and its invalid entries give NaN maps.
  *   out: device pointer, 6 * count floats, map-major: out[m * count + i] with
  *        m = dA, ellipticity1, ellipticity2, rotation, deflection1, deflection2
  *        (.cu:577-582).

@@ -1565,8 +1565,9 @@ int trace_entry(const char* who, bool no_alloc, srt_context* ctx, int64_t nside,
     if (nside <= 0 || nside > (1 << 20)) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "nside out of range");
     if (n_steps < 0) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "n_steps < 0");
     const int64_t npix = 12 * nside * nside;
-    if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "count out of range");
-    if (!pix_list && (pix_begin < 0 || pix_begin + count > npix))
+    // a pixel list may repeat pixels (any count); a RING range lies within the sky
+    if (count < 0) return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "count < 0");
+    if (!pix_list && (count > npix || pix_begin < 0 || pix_begin > npix - count))
         return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "pixel range outside the sky");
     if (ray_order != SRT_ORDER_NESTED && ray_order != SRT_ORDER_LINEAR)
         return fail(SRT_ERR_INVALID_ARGUMENT, w + ": " + "unknown ray order");
@@ -1695,15 +1696,16 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
     if (n_steps < 1 || checkpoint_every < 1)
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: n_steps and checkpoint_every must be >= 1");
     const int64_t npix = 12 * nside * nside;
-    if (count < 0 || count > npix) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: count out of range");
-    if (!pix_list && (pix_begin < 0 || pix_begin + count > npix))
+    // a pixel list may repeat pixels (any count); a RING range lies within the sky
+    if (count < 0) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: count < 0");
+    if (!pix_list && (count > npix || pix_begin < 0 || pix_begin > npix - count))
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: pixel range outside the sky");
     if (map_grad && (!grad_phi || !grad_shift_xyz))
         return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: gradient outputs are null");
     if (!map_grad && !maps64) return fail(SRT_ERR_INVALID_ARGUMENT, "srt_trace_adjoint: nothing to compute");
     // at most 2^32 - 64 rays (rounded to whole wavefronts, below 2^32 lanes; the
     // launches themselves are split into chunks far smaller than that)
-    if ((count + 63) / 64 * 64 >= (int64_t(1) << 32))
+    if (count > (int64_t(1) << 32) - 64)
         return fail(SRT_ERR_UNSUPPORTED, "srt_trace_adjoint: more rays than one call holds (2^32 lanes)");
     SRT_HIP(hipSetDevice(ctx->device));
     const int K = std::min(checkpoint_every, n_steps);
