@@ -1,18 +1,21 @@
 """Parity over seeded random configurations: the HIP kernels against the host
 FP64 oracle on shapes the fixed tests do not pin one by one.
 
-Each of 24 cases draws (numpy default_rng(100 + case)): a grid edge in
+Each of 30 cases draws (numpy default_rng(100 + case)): a grid edge in
 [4, 40] (odd and non-power-of-two edges included), Nside in [1, 12] (most not
 powers of two), n_steps in [0, 3n] (rays that cross the periodic box up to
 three times), an observer anywhere in [-n, 2n]^3 or the default centre, the
-arithmetic (FP64, or mixed on the padded grids or on bicubic records of a
-random brick edge), the lane order (LINEAR or NESTED, sorted on the device
+arithmetic (FP64; FP64 on the reference's binary16 x 2^27 shift storage,
+against the oracle on the same codes; mixed on the padded grids or on
+bicubic records of a random brick edge; the reference's float/double mix,
+against the oracle's REF form), the lane order (LINEAR or NESTED, sorted on the device
 where Nside is not a power of two), and the rays (the full sky, one RING
 batch, or a random pixel list with repeated and invalid entries).
 
 Bars: FP64 final states within 1e-10 of each component's scale and maps within
 2 float ulps; mixed dA within 1e-5 relative, shear rms(|d eps|)/rms(|eps|)
-<= 1e-4, rotation and deflection rms(d)/rms <= 1e-3; with 0 steps every map
+<= 1e-4, rotation and deflection rms(d)/rms <= 1e-3; REF dA within 1e-6
+relative and the other maps within 2e-5 of their rms; with 0 steps every map
 exactly 0; invalid pixel-list entries give NaN maps.  The cases are fixed by
 their seeds and the kernels are deterministic, so the outcome is too.
 """
@@ -21,7 +24,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CASES = 24
+CASES = 30
 
 
 def draw(case):
@@ -30,7 +33,7 @@ def draw(case):
     nside = int(rng.integers(1, 13))
     steps = int(rng.integers(0, 3 * n + 1))
     observer = None if rng.random() < 0.25 else rng.uniform(-n, 2 * n, 3)
-    kind = ["fp64", "mixed_taps", "mixed_records"][case % 3]
+    kind = ["fp64", "mixed_taps", "mixed_records", "ref", "fp64_f16"][case % 5]
     brick = int(rng.choice([1, 2, 4, 8, 16, 32]))
     order = ["linear", "nested"][int(rng.integers(0, 2))]
     npix = 12 * nside * nside
@@ -56,8 +59,10 @@ def test_random_configuration(srt, oracle, fields, case):
     tr = srt.Tracer(0)
     if c["kind"] == "mixed_records":
         tr.set_layout(srt.LAYOUT_BICUBIC, c["brick"])
-    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)))
-    prec = srt.PREC_FP64 if c["kind"] == "fp64" else srt.PREC_MIXED
+    f16 = c["kind"] == "fp64_f16"
+    tr.upload(phi, np.ascontiguousarray(np.moveaxis(shift, 0, -1)),
+              shift_format=srt.SHIFT_F16_SCALED if f16 else srt.SHIFT_F32)
+    prec = {"ref": srt.PREC_REF, "fp64": srt.PREC_FP64, "fp64_f16": srt.PREC_FP64}.get(c["kind"], srt.PREC_MIXED)
     order = srt.ORDER_NESTED if c["order"] == "nested" else srt.ORDER_LINEAR
     maps, state = tr.trace(c["nside"], c["steps"], batch=c["batch"], pix=c["pix"], observer=c["observer"],
                            precision=prec, order=order, return_state=True)
@@ -72,16 +77,25 @@ def test_random_configuration(srt, oracle, fields, case):
     maps, state, pix = maps[:, valid], state[valid], pix[valid]
     if pix.size == 0:
         return
-    ref, _, ref_state = oracle.trace(oracle.Fields(phi, shift), c["nside"], pix, c["steps"],
-                                     observer=c["observer"], form=oracle.FORM_FP64, return_state=True)
+    F = oracle.Fields(phi, shift, b16=oracle.shift_to_f16(shift) if f16 else None)
+    form = oracle.FORM_REF if c["kind"] == "ref" else oracle.FORM_FP64
+    ref, _, ref_state = oracle.trace(F, c["nside"], pix, c["steps"], observer=c["observer"], form=form,
+                                     return_state=True)
     if c["steps"] == 0:
         assert np.all(maps == 0) and np.all(ref == 0)
         return
-    if c["kind"] == "fp64":
+    if c["kind"] in ("fp64", "fp64_f16"):
         scale = np.abs(ref_state).max(axis=0)
         scale[scale == 0] = 1.0
         assert (np.abs(state - ref_state) / scale).max() < 1e-10
         np.testing.assert_allclose(maps, ref, rtol=2.5e-7, atol=1e-30)
+        return
+    if c["kind"] == "ref":
+        assert np.max(np.abs(maps[0].astype(np.float64) - ref[0]) / np.abs(ref[0])) < 1e-6
+        for m in range(1, 6):
+            rms = np.sqrt(np.mean(ref[m].astype(np.float64) ** 2))
+            if rms > 0:
+                assert np.abs(maps[m].astype(np.float64) - ref[m]).max() / rms < 2e-5, m
         return
     assert np.max(np.abs(maps[0].astype(np.float64) - ref[0]) / np.abs(ref[0])) <= 1e-5
     de = np.hypot(maps[1].astype(np.float64) - ref[1], maps[2].astype(np.float64) - ref[2])
