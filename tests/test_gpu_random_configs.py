@@ -16,8 +16,12 @@ Bars: FP64 final states within 1e-10 of each component's scale and maps within
 2 float ulps; mixed dA within 1e-5 relative, shear rms(|d eps|)/rms(|eps|)
 <= 1e-4, rotation and deflection rms(d)/rms <= 1e-3; REF dA within 1e-6
 relative and the other maps within 2e-5 of their rms; with 0 steps every map
-exactly 0; invalid pixel-list entries give NaN maps.  The cases are fixed by
-their seeds and the kernels are deterministic, so the outcome is too.
+exactly 0; invalid pixel-list entries give NaN maps.
+
+Ten random batch plans (default_rng(500 + case)) check the one-launch path
+against single traces of the same items, bit for bit where the plan promises
+it.  The cases are fixed by their seeds and the kernels are deterministic, so
+the outcome is too.
 """
 import numpy as np
 import pytest
@@ -106,3 +110,105 @@ def test_random_configuration(srt, oracle, fields, case):
         rms = np.sqrt(np.mean(ref[m].astype(np.float64) ** 2))
         if rms > 0:
             assert np.sqrt(np.mean((maps[m].astype(np.float64) - ref[m]) ** 2)) / rms <= 1e-3, m
+
+
+PLANS = 10
+
+
+def draw_plan(case):
+    rng = np.random.default_rng(500 + case)
+    contexts = [dict(n=int(rng.integers(4, 41)), records=bool(rng.random() < 0.6), brick=int(rng.choice([1, 4, 16])),
+                     seed=2000 + 10 * case + k) for k in range(int(rng.integers(1, 5)))]
+    items = []
+    for _ in range(int(rng.integers(1, 9))):
+        k = int(rng.integers(0, len(contexts)))
+        nside = int(rng.integers(1, 13))
+        npix = 12 * nside * nside
+        count = int(rng.integers(0, npix + 1))
+        pix_begin = int(rng.integers(0, npix - count + 1))
+        n = contexts[k]["n"]
+        items.append(dict(ctx=k, nside=nside, pix_begin=pix_begin, count=count,
+                          n_steps=int(rng.integers(0, 3 * n + 1)),
+                          observer=None if rng.random() < 0.3 else rng.uniform(-n, 2 * n, 3)))
+    precision = ["mixed", "fp64", "ref"][case % 3]
+    schedule = int(rng.integers(0, 2))
+    order = ["linear", "nested"][int(rng.integers(0, 2))]
+    groups = int(rng.choice([1, 3, 8]))
+    return dict(contexts=contexts, items=items, precision=precision, schedule=schedule, order=order,
+                groups=groups)
+
+
+def close_to_mixed_bars(got, exp):
+    """Two mixed traces of one item, each within the mixed bars of FP64: dA within 2e-5 relative, every other map
+    within 2e-3 of its rms (maps that are 0 or NaN everywhere, as with 0 steps, must agree exactly)."""
+    ok = ~np.isnan(exp)
+    for m in range(6):
+        g, e = got[m][ok[m]], exp[m][ok[m]]
+        rms = np.sqrt(np.mean(e ** 2)) if e.size else 0.0
+        if rms == 0:
+            assert np.array_equal(g, e), m
+        elif m == 0:
+            assert np.max(np.abs(g - e) / np.abs(e)) <= 2e-5
+        else:
+            assert np.sqrt(np.mean((g - e) ** 2)) / rms <= 2e-3, m
+
+
+@pytest.mark.parametrize("case", range(PLANS))
+def test_random_plan(srt, fields, case):
+    """A random plan: 1-4 snapshots (edges 4-40, bicubic records on some), 1-8 items over them (several items may
+    share a snapshot; arbitrary RING ranges, empty ones included; 0 to 3n steps; observers anywhere), FP64, mixed
+    or REF, either schedule, LINEAR or NESTED lanes, 1, 3 or 8 tile ranges, launched directly and replayed from a
+    HIP graph: every item equals its own srt_trace bit for bit.  The one exception is a mixed plan whose snapshots
+    differ in layout: it traces every item on the padded grids, so the items on snapshots with records are held to
+    twice the mixed form's bars against FP64 (each trace is within them) instead."""
+    import torch
+    p = draw_plan(case)
+    print(f"[gpu] random plan {case}: {len(p['contexts'])} snapshots {[c['n'] for c in p['contexts']]}, "
+          f"{len(p['items'])} items, {p['precision']}, schedule {p['schedule']}, {p['order']}, "
+          f"{p['groups']} groups", flush=True)
+    dev = torch.device("cuda:0")
+    tracers = []
+    for c in p["contexts"]:
+        phi, shift = fields.snapshot(c["n"], c["seed"])
+        tr = srt.Tracer(0)
+        if c["records"]:
+            tr.set_layout(srt.LAYOUT_BICUBIC, c["brick"])
+        tr.upload(phi.numpy(), fields.interleave_shift(shift).numpy())
+        tracers.append(tr)
+    prec = srt.PRECISIONS[p["precision"]]
+    items = [dict(tracer=tracers[it["ctx"]], nside=it["nside"], pix_begin=it["pix_begin"], count=it["count"],
+                  n_steps=it["n_steps"], observer=it["observer"],
+                  out=torch.full((6 * max(it["count"], 1),), float("nan"), dtype=torch.float32, device=dev))
+             for it in p["items"]]
+    want = []
+    for it in items:
+        o = torch.empty_like(it["out"])
+        it["tracer"].trace_device(it["nside"], it["n_steps"], o, pix_begin=it["pix_begin"], count=it["count"],
+                                  observer=it["observer"], precision=prec)
+        want.append(o)
+    torch.cuda.synchronize()
+    plan = srt.Plan(items, precision=prec)
+    plan.set_schedule(srt.SCHEDULE_PERSISTENT if p["schedule"] == 0 else srt.SCHEDULE_DISPATCH)
+    plan.set_order(srt.ORDER_NESTED if p["order"] == "nested" else srt.ORDER_LINEAR, p["groups"])
+    layouts = {c["records"] for c in p["contexts"]}
+    loose = [p["precision"] == "mixed" and len(layouts) > 1 and p["contexts"][it["ctx"]]["records"]
+             for it in p["items"]]
+    stream = torch.cuda.Stream()
+    for graph in (False, True, True):
+        for it in items:
+            it["out"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.launch(stream=stream, graph=graph)
+        stream.synchronize()
+        for it, w, lo in zip(items, want, loose):
+            c = 6 * it["count"]
+            got, exp = it["out"][:c], w[:c]  # 0-step deflections are 0/0 = NaN in both
+            same = (got == exp) | (torch.isnan(got) & torch.isnan(exp))
+            if lo:
+                assert bool((torch.isnan(got) == torch.isnan(exp)).all())
+                close_to_mixed_bars(got.view(6, -1).double().cpu().numpy(), exp.view(6, -1).double().cpu().numpy())
+            else:
+                assert bool(same.all()), (graph, it["nside"], it["pix_begin"], it["count"])
+    plan.close()
+    for tr in tracers:
+        tr.close()
