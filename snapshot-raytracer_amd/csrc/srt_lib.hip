@@ -1720,7 +1720,7 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
                  n_rec = map_grad ? (size_t)chunk * (size_t)(4 * K) * kAdjRec : 0, n_gr = map_grad ? 4 * pvox : 0;
     const size_t need = (n_ck + n_lam + n_rec + n_gr) * sizeof(double);
     if (need > ctx->adj_bytes) {
-        SRT_HIP(hipStreamSynchronize((hipStream_t)stream));
+        SRT_HIP(hipDeviceSynchronize());  // an earlier call, on any stream, may still use the old buffers
         if (ctx->adj_buf) SRT_HIP(hipFree(ctx->adj_buf));
         ctx->adj_buf = nullptr;
         ctx->adj_bytes = 0;
@@ -2070,7 +2070,7 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
         if (it.ctx->device != device)
             return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: contexts on different devices");
         if (it.nside <= 0 || it.nside > (1 << 20) || it.n_steps < 0 || it.count < 0 || it.pix_begin < 0 ||
-            it.pix_begin + it.count > 12 * it.nside * it.nside)
+            it.count > 12 * it.nside * it.nside || it.pix_begin > 12 * it.nside * it.nside - it.count)
             return fail(SRT_ERR_INVALID_ARGUMENT, "srt_plan_create: item " + std::to_string(i) + " out of range");
     }
     SRT_HIP(hipSetDevice(device));
