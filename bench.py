@@ -34,6 +34,18 @@ FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12793.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
 HBM_BYTES_PER_LAUNCH = {"mixed": 28.44e9, "mixed_taps": 108.4e9, "fp64": 7.87e9}
+# The gather path per launch of each kernel, from rocprofv3 TCP, TA and TCC
+# counters of one launch of this workload (same profile): L1 (TCP) cache-line
+# tag accesses (TCP_TOTAL_CACHE_ACCESSES_sum), L2 read bytes
+# (TCC_READ_SECTORS_sum x 32 B) and the share of cycles the texture-address
+# unit was busy (TA_BUSY_avr over GRBM_GUI_ACTIVE / 8 XCDs).
+GATHER_PER_LAUNCH = {"mixed": {"l1_lines": 2.7518e11, "l2_read_bytes": 6.3734e11, "ta_busy": 0.759},
+                     "mixed_taps": {"l1_lines": 3.2305e11, "l2_read_bytes": 1.38854e12, "ta_busy": 0.725},
+                     "fp64": {"l1_lines": 5.6282e11, "l2_read_bytes": 3.4406e11, "ta_busy": 0.511}}
+# L2 read peak: 8 XCDs x 16 channels (TCC_CYCLE_sum / cycles = 128 channels) x 128 B per channel and clock at
+# 2.4 GHz; the CUs and the clock of the L1 rate
+L2_PEAK_TBPS = 128 * 128 * 2.4e9 / 1e12
+CUS, CLOCK_HZ = 256, 2.4e9
 # Algorithmic flops per ray-step, the roofline's work constant: the operation
 # count of a work-efficient form of the reference's mathematics (per axis the
 # Catmull-Rom weight vectors once, weight sets shared between the four fields of
@@ -693,6 +705,15 @@ def main():
             roofline["executed"] = {"flops_per_ray_step": FLOPS_PER_RAY_STEP[head], "tflops": ex,
                                     "frac_of_peak": ex / PEAK_TFLOPS[head],
                                     "frac_of_unpacked_fp32_rate": ex / UNPACKED_FP32_TFLOPS}
+        if head in GATHER_PER_LAUNCH:
+            g, t = GATHER_PER_LAUNCH[head], r["kernel_ms_avg"] / 1000.0
+            roofline["gather"] = {
+                "l1_lines_per_ray_step": g["l1_lines"] / rs_launch,
+                "l1_lines_per_cu_clock": g["l1_lines"] / (CUS * CLOCK_HZ * t),
+                "ta_busy": g["ta_busy"],
+                "l2_read_bytes": g["l2_read_bytes"], "l2_read_TBps": g["l2_read_bytes"] / t / 1e12,
+                "l2_peak_TBps": L2_PEAK_TBPS, "l2_frac": g["l2_read_bytes"] / t / 1e12 / L2_PEAK_TBPS,
+                "stencil_bytes_per_ray_step": 4096, "stencil_TBps": 4096 * rs_launch / t / 1e12}
         if "fp64" in results and head != "fp64":
             a64 = ALGO_FLOPS_PER_RAY_STEP * rs_launch / (results["fp64"]["kernel_ms_avg"] / 1000.0) / 1e12
             roofline["fp64_kernel"] = {"achieved": a64, "peak": PEAK_TFLOPS["fp64"], "frac": a64 / PEAK_TFLOPS["fp64"]}
