@@ -69,14 +69,33 @@ def test_cli_batch_masks_and_oracle(srt, oracle, files):
         np.testing.assert_allclose(got[lo:lo + cnt], ref[m], rtol=3e-7, atol=1e-30)
 
 
-def test_cli_reference_storage_mode(files):
-    d = files[0]
-    r = run(d, ["4", "6"], {"SRT_PRECISION": "ref", "SRT_SHIFT": "f16"})
+def test_cli_reference_storage_mode(srt, oracle, files):
+    """SRT_PRECISION=ref with SRT_SHIFT=f16, the reference's arithmetic on its binary16 x 2^27 shift storage
+    (.cu:585-597, 241-244): the FITS maps equal the library's trace in that mode bit for bit, and the oracle's REF
+    form on the same binary16 codes within the REF bars (dA 1e-6 relative, other maps 2e-5 of their rms).  Unknown
+    values of the three variables exit with status 255."""
+    from snapshot_raytracer_amd import snapshot_io
+    d, phi, shift, shift_il = files
+    nside, steps = 4, 6
+    r = run(d, [str(nside), str(steps)], {"SRT_PRECISION": "ref", "SRT_SHIFT": "f16"})
     assert r.returncode == 0, r.stderr
-    r = run(d, ["4", "6"], {"SRT_PRECISION": "bogus"})
-    assert r.returncode == 255
-    r = run(d, ["4", "6"], {"SRT_LAYOUT": "bogus"})
-    assert r.returncode == 255
+    tr = srt.Tracer(0)
+    tr.upload(phi, shift_il, shift_format=srt.SHIFT_F16_SCALED)
+    lib = tr.trace(nside, steps, precision=srt.PREC_REF)
+    tr.close()
+    F = oracle.Fields(phi, shift, b16=oracle.shift_to_f16(shift))
+    ref, _ = oracle.trace(F, nside, np.arange(12 * nside * nside), steps, form=oracle.FORM_REF)
+    for m, name in enumerate(NAMES):
+        got, _ = snapshot_io.read_healpix_fits(str(d / f"{name}_batch0.fits"))
+        assert np.array_equal(got, lib[m]), name
+        g, e = got.astype(np.float64), ref[m].astype(np.float64)
+        if m == 0:
+            assert np.max(np.abs(g - e) / np.abs(e)) < 1e-6
+        else:
+            assert np.max(np.abs(g - e)) / np.sqrt(np.mean(e ** 2)) < 2e-5, name
+    for var in ("SRT_PRECISION", "SRT_SHIFT", "SRT_LAYOUT"):
+        r = run(d, ["4", "6"], {var: "bogus"})
+        assert r.returncode == 255, var
 
 
 def test_cli_records_that_do_not_fit(srt, files):
