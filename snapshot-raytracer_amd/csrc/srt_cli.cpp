@@ -29,7 +29,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <memory>
+#include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/srt.h"
@@ -114,24 +117,57 @@ int main(int argc, char* argv[]) {
     using clock = std::chrono::steady_clock;
     const auto t_begin = clock::now();
     auto seconds = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    // The HIP runtime and the library context start on a second thread while
+    // the snapshot is read: runtime start-up takes a few tenths of a second,
+    // about as long as reading a 512^3 snapshot.
+    srt_context* ctx = nullptr;
+    int create_rc = SRT_OK;
+    std::string create_err;  // srt_last_error is per thread
+    auto start_device = [&] {
+        create_rc = srt_create((int)device, &ctx);
+        if (create_rc != SRT_OK) create_err = srt_last_error();
+    };
+    std::thread starter;
+    try {
+        starter = std::thread(start_device);
+    } catch (...) {
+        start_device();
+    }
+    auto stop_device = [&] {
+        if (starter.joinable()) starter.join();
+        srt_destroy(ctx);
+        ctx = nullptr;
+    };
+    // the readers fill buffers that start uninitialised (no pass that zeroes
+    // gigabytes first); a failed allocation leaves the buffer null
+    auto sink_into = [](std::unique_ptr<float[]>& buf) {
+        return srt_io::FloatSink([&buf](uint64_t count) -> float* {
+            buf.reset(new (std::nothrow) float[count]);
+            return buf.get();
+        });
+    };
     std::cout << "Loading potential data from HDF5 file " << argv[1] << " ..." << std::endl;
     std::string err;
-    std::vector<float> phi;
+    std::unique_ptr<float[]> phi, shift;
     std::vector<uint64_t> dims;
-    if (!srt_io::read_scalar_field(argv[1], "/data", phi, dims, err)) {
-        std::cerr << err << std::endl;
+    const bool read_phi = srt_io::read_scalar_field(argv[1], "/data", sink_into(phi), dims, err);
+    if (!read_phi || !phi) {
+        std::cerr << (read_phi ? "Out of host memory for the potential" : err) << std::endl;
+        stop_device();
         return -1;
     }
     std::cout << "Data dimensions: " << dims[0] << " x " << dims[1] << " x " << dims[2] << std::endl;
     std::cout << "Loading B field data from HDF5 file " << argv[2] << " ..." << std::endl;
-    std::vector<float> shift;
     std::vector<uint64_t> bdims;
-    if (!srt_io::read_vector_field(argv[2], "/data", shift, bdims, err)) {
-        std::cerr << err << std::endl;
+    const bool read_shift = srt_io::read_vector_field(argv[2], "/data", sink_into(shift), bdims, err);
+    if (!read_shift || !shift) {
+        std::cerr << (read_shift ? "Out of host memory for the B field" : err) << std::endl;
+        stop_device();
         return -1;
     }
     if (bdims != dims) {
         std::cerr << "Potential and B field dimensions differ" << std::endl;
+        stop_device();
         return -1;
     }
 
@@ -142,24 +178,29 @@ int main(int argc, char* argv[]) {
         count = nside * nside;
         if (istart >= npix) {
             std::cerr << "Batch exceeds total number of pixels" << std::endl;
+            stop_device();
             return -1;
         }
     }
 
     const auto t_read = clock::now();
     std::cout << "Allocating device memory and copying data to device..." << std::endl;
-    srt_context* ctx = nullptr;
-    int rc = srt_create((int)device, &ctx);
-    if (rc == SRT_OK)
-        rc = srt_upload(ctx, phi.data(), shift.data(), (int64_t)dims[2], (int64_t)dims[1], (int64_t)dims[0], 0,
-                        shift_s == "f16" ? SRT_SHIFT_F16_SCALED : SRT_SHIFT_F32);
+    if (starter.joinable()) starter.join();
+    int rc = create_rc;
     if (rc != SRT_OK) {
-        std::cerr << "Device setup failed: " << srt_strerror(rc) << ": " << srt_last_error() << std::endl;
-        srt_destroy(ctx);
+        std::cerr << "Device setup failed: " << srt_strerror(rc) << ": " << create_err << std::endl;
+        stop_device();
         return -1;
     }
-    std::vector<float>().swap(phi);
-    std::vector<float>().swap(shift);
+    rc = srt_upload(ctx, phi.get(), shift.get(), (int64_t)dims[2], (int64_t)dims[1], (int64_t)dims[0], 0,
+                    shift_s == "f16" ? SRT_SHIFT_F16_SCALED : SRT_SHIFT_F32);
+    if (rc != SRT_OK) {
+        std::cerr << "Device setup failed: " << srt_strerror(rc) << ": " << srt_last_error() << std::endl;
+        stop_device();
+        return -1;
+    }
+    phi.reset();
+    shift.reset();
     if (precision == SRT_PREC_MIXED && layout_s != "taps") {
         rc = srt_set_layout(ctx, SRT_LAYOUT_BICUBIC, 16);
         if (rc == SRT_ERR_OUT_OF_MEMORY && layout_s == "auto") {

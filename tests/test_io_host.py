@@ -65,6 +65,82 @@ def test_hdf5_round_trip(sio, iolib, tmp_path):
     assert np.array_equal(got, (phi.astype(np.float64) * 1.000000001).astype(np.float32))
 
 
+def write_h5(sio, path, arr, file_type, mem_type, chunk=None, userblock=0):
+    """One rank-3 dataset /data with optional chunked layout and user block, through the HDF5 C library."""
+    L = sio._lib()
+    hid = ctypes.c_int64
+    L.H5Pcreate.restype = hid
+    L.H5Pcreate.argtypes = [hid]
+    L.H5Pset_chunk.argtypes = [hid, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+    L.H5Pset_userblock.argtypes = [hid, ctypes.c_uint64]
+    L.H5Pclose.argtypes = [hid]
+    fcpl = dcpl = 0
+    if userblock:
+        fcpl = L.H5Pcreate(sio._global("H5P_CLS_FILE_CREATE_ID_g"))
+        assert L.H5Pset_userblock(fcpl, userblock) >= 0
+    if chunk:
+        dcpl = L.H5Pcreate(sio._global("H5P_CLS_DATASET_CREATE_ID_g"))
+        assert L.H5Pset_chunk(dcpl, 3, (ctypes.c_uint64 * 3)(*chunk)) >= 0
+    fid = L.H5Fcreate(path.encode(), 0x0002, fcpl, 0)
+    assert fid >= 0
+    sid = L.H5Screate_simple(3, (ctypes.c_uint64 * 3)(*arr.shape[:3]), None)
+    did = L.H5Dcreate2(fid, b"/data", file_type, sid, 0, dcpl, 0)
+    assert did >= 0
+    assert L.H5Dwrite(did, mem_type, 0, 0, 0, np.ascontiguousarray(arr).ctypes.data_as(ctypes.c_void_p)) >= 0
+    L.H5Dclose(did)
+    L.H5Sclose(sid)
+    L.H5Fclose(fid)
+    for pl in (fcpl, dcpl):
+        if pl:
+            L.H5Pclose(pl)
+
+
+def test_hdf5_raw_and_converted_reads(sio, iolib, tmp_path):
+    """Contiguous little-endian float datasets are read straight from the file by several threads (160^3: a
+    16 MB potential, a 49 MB shift); big-endian floats, chunked datasets and files with a user block go through
+    H5Dread.  Every way returns the same floats."""
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal((160, 160, 160)).astype(np.float32)
+    shift = rng.standard_normal((160, 160, 160, 3)).astype(np.float32)
+    sio.write_potential(str(tmp_path / "p.h5"), phi)
+    sio.write_shift(str(tmp_path / "b.h5"), shift)
+    got, _ = read_field(iolib, "srtio_read_scalar_field", str(tmp_path / "p.h5"), 1)
+    assert np.array_equal(got, phi)
+    got, _ = read_field(iolib, "srtio_read_vector_field", str(tmp_path / "b.h5"), 3)
+    assert np.array_equal(got, shift)
+    L = sio._lib()
+    native, be = sio._global("H5T_NATIVE_FLOAT_g"), sio._global("H5T_IEEE_F32BE_g")
+    small, small3 = phi[:24, :20, :16], shift[:24, :20, :16]
+    three = (ctypes.c_uint64 * 1)(3)
+    arr_native, arr_be = L.H5Tarray_create2(native, 1, three), L.H5Tarray_create2(be, 1, three)
+    cases = [("be.h5", small, be, native, None, 0, 1), ("chunk.h5", small, native, native, (8, 10, 16), 0, 1),
+             ("ub.h5", small, native, native, None, 512, 1), ("be3.h5", small3, arr_be, arr_native, None, 0, 3),
+             ("chunk3.h5", small3, arr_native, arr_native, (12, 5, 16), 0, 3),
+             ("ub3.h5", small3, arr_native, arr_native, None, 1024, 3)]
+    for name, arr, ft, mt, chunk, ub, comps in cases:
+        write_h5(sio, str(tmp_path / name), arr, ft, mt, chunk, ub)
+        fn = "srtio_read_scalar_field" if comps == 1 else "srtio_read_vector_field"
+        got, e = read_field(iolib, fn, str(tmp_path / name), comps)
+        assert e == "" and np.array_equal(got, arr), name
+    L.H5Tclose(arr_native)
+    L.H5Tclose(arr_be)
+
+
+def test_hdf5_reader_buffer_checks(sio, iolib, tmp_path):
+    """A null buffer returns the dimensions only; a buffer smaller than the dataset is refused untouched."""
+    phi = np.arange(4 * 5 * 6, dtype=np.float32).reshape(4, 5, 6)
+    sio.write_potential(str(tmp_path / "p.h5"), phi)
+    dims = np.zeros(3, np.uint64)
+    err = ctypes.create_string_buffer(256)
+    assert iolib.srtio_read_scalar_field(str(tmp_path / "p.h5").encode(), b"/data", None, 0, dims.ctypes.data,
+                                         err, 256) == 0
+    assert list(dims) == [4, 5, 6]
+    out = np.full(phi.size - 1, -7.0, np.float32)
+    assert iolib.srtio_read_scalar_field(str(tmp_path / "p.h5").encode(), b"/data", out.ctypes.data, out.size,
+                                         dims.ctypes.data, err, 256) == 1
+    assert err.value.decode() == "output buffer too small" and np.all(out == -7.0)
+
+
 def test_hdf5_reader_errors(sio, iolib, tmp_path):
     _, e = read_field(iolib, "srtio_read_scalar_field", str(tmp_path / "missing.h5"), 1)
     assert e.startswith("Failed to open HDF5 file")
