@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <iostream>
 #include <memory>
 #include <new>
@@ -242,13 +243,35 @@ int main(int argc, char* argv[]) {
     const char* names[6] = {"dAmap", "ellipticity1map", "ellipticity2map",
                             "rotationmap", "deflection1map", "deflection2map"};
     const int64_t file_batch = batch < 0 ? 0 : batch;
-    std::vector<float> outmap((size_t)npix, 0.0f);
+    // the six files are independent: one thread writes each (a map whose
+    // thread cannot be started is written by this one afterwards)
+    std::string errs[6];
+    bool written[6] = {};
+    auto write_map = [&](int m) {
+        try {
+            std::vector<float> outmap((size_t)npix, 0.0f);
+            std::memcpy(outmap.data() + istart, maps.data() + (size_t)m * count, (size_t)count * sizeof(float));
+            char fname[256];
+            std::snprintf(fname, sizeof(fname), "%s_batch%lld.fits", names[m], (long long)file_batch);
+            written[m] = srt_io::write_healpix_map(fname, outmap.data(), nside, false, "C", errs[m]);
+        } catch (const std::exception& e) {
+            errs[m] = std::string("writing ") + names[m] + ": " + e.what();
+        }
+    };
+    std::vector<std::thread> writers;
+    std::vector<int> inline_maps;
     for (int m = 0; m < 6; ++m) {
-        std::memcpy(outmap.data() + istart, maps.data() + (size_t)m * count, (size_t)count * sizeof(float));
-        char fname[256];
-        std::snprintf(fname, sizeof(fname), "%s_batch%lld.fits", names[m], (long long)file_batch);
-        if (!srt_io::write_healpix_map(fname, outmap.data(), nside, false, "C", err)) {
-            std::cerr << err << std::endl;
+        try {
+            writers.emplace_back(write_map, m);
+        } catch (...) {
+            inline_maps.push_back(m);
+        }
+    }
+    for (int m : inline_maps) write_map(m);
+    for (auto& t : writers) t.join();
+    for (int m = 0; m < 6; ++m) {
+        if (!written[m]) {
+            std::cerr << errs[m] << std::endl;
             return -1;
         }
     }
