@@ -304,7 +304,9 @@ def workload_adjoint(args, pkg, fields, dev):
         i, j, k = (int(a) for a in v)
         comp = len(checks) % 4  # 0 = potential, 1..3 = shift components
         base = phi_h if comp == 0 else shift_h[..., comp - 1]
-        h = 1e-3 * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
+        # steps: 1e-3 x rms for the potential, 1e-1 x rms for the shift, the steps of least difference error
+        # (a smaller shift step only adds the rounding noise of the perturbed traces, which falls as 1/h)
+        h = (1e-3 if comp == 0 else 1e-1) * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
         vals = []
         for sgn in (1, -1):
             p2, s2 = phi_h.copy(), shift_h.copy()

@@ -6,7 +6,9 @@ Workload: periodic 32^3 snapshot (seed 31), Nside 8 (768 rays), 12 steps,
 checkpoints every 4 steps; loss = sum over pixels of (eps1 - t1)^2 +
 (eps2 - t2)^2 + (dA - tA)^2 with targets from a second seed's maps plus
 Gaussian noise (sigma = 0.01 x map rms).  Finite differences use the step
-1e-3 x field rms on voxels within 8 cells of the observer; the perturbed
+1e-3 x the potential's rms and 1e-1 x the shift's rms (FD_STEP, the steps of
+least difference error, as measured on the 128^3 workload in
+test_gpu_adjoint_128cube.py) on voxels within 8 cells of the observer; the perturbed
 value is the float32 the grid stores, the difference quotient uses the
 exact float32 step, and L(+) - L(-) is summed as (m+ - m-)(m+ + m- - 2t) per
 pixel.  Bar: 1e-6 relative to the largest checked gradient of each grid.
@@ -17,6 +19,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 N, NSIDE, STEPS, K = 32, 8, 12, 4
+FD_STEP = {"phi": 1e-3, 0: 1e-1, 1: 1e-1, 2: 1e-1}  # fractions of each field's rms
 
 
 @pytest.fixture(scope="module")
@@ -79,7 +82,7 @@ def test_adjoint_matches_central_differences(srt, setup, fields):
     checks = []
     for comp, vs in (("phi", voxels(rng, 8)), (0, voxels(rng, 4)), (1, voxels(rng, 4)), (2, voxels(rng, 4))):
         base = phi if comp == "phi" else shift[comp]
-        h = 1e-3 * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
+        h = FD_STEP[comp] * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
         for (i, j, k) in vs:
             vals = []
             for sgn in (+1, -1):
@@ -145,7 +148,7 @@ def test_adjoint_vs_oracle_differences(srt, oracle, setup):
     res = []
     for comp, (i, j, k) in [("phi", v) for v in voxels(rng, 3)] + [(1, v) for v in voxels(rng, 2)]:
         base = phi if comp == "phi" else shift[comp]
-        h = 1e-3 * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
+        h = FD_STEP[comp] * float(np.sqrt(np.mean(base.astype(np.float64) ** 2)))
         vals = []
         for sgn in (+1, -1):
             p = phi.copy()
