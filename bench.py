@@ -68,7 +68,7 @@ UNPACKED_FP32_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one 128^3 / Nside 64 / 64-step / K = 16 gradient pass: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # summed over its dispatches (profiles/adjoint_kernels_128cube_nside64_kernel_trace.md)
-ADJOINT_HBM_BYTES_PER_PASS = 36.9e9
+ADJOINT_HBM_BYTES_PER_PASS = 39.1e9
 # doubles of one stage record of the adjoint (state, then value, gradient, Hessian and third derivatives of
 # the four fields) and of the interpolation adjoints one stage writes (10 per field)
 ADJ_STAGE_RECORD_DOUBLES = 19 + 4 * 20

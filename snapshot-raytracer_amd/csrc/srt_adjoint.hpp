@@ -6,9 +6,9 @@
 // Everything here is FP64.  The work is split so that each part runs with the
 // parallelism it has:
 //   * the interpolations of a ray's four fields (potential, B_x, B_y, B_z) are
-//     independent: field_stencil / field_interp evaluate one field, so four
-//     lanes can serve one ray (value, gradient, Hessian and, for the reverse
-//     sweep, the third derivatives);
+//     independent: field_stencil / field_interp evaluate one field, so
+//     several lanes can serve one ray (value, gradient, Hessian and, for the
+//     reverse sweep, the third derivatives);
 //   * the right-hand side is restated with its intermediates kept
 //     (rhs_forward_from) from the four fields' interpolants, and its
 //     vector-Jacobian product (rhs_vjp_bars) is pure algebra: it takes the

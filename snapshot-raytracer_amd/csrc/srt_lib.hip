@@ -647,11 +647,21 @@ __global__ __launch_bounds__(trace_lanes<F>(), F::kWavesPerSimd) void batch_disp
 //      algebra), writing each field's interpolation adjoints into the records;
 //   4. adj_scatter_kernel: the transpose of each record's 64-tap contraction,
 //      added to padded FP64 gradient grids.
-// Kernels 1 and 2 give a ray four lanes, one per field (potential, B_x, B_y,
-// B_z): the lanes interpolate their fields, exchange the 40 interpolants
-// through LDS and evaluate the right-hand side together; kernel 4 runs one
-// wavefront per 64 rays and (step, stage, field).
-constexpr int kAdjRays = 16;                    // rays per 64-lane workgroup of adj_forward_kernel
+// Kernels 1 and 2 give a ray two lanes, each interpolating two of its fields
+// (potential, B_x, B_y, B_z): the lanes exchange the 40 interpolants through
+// LDS and evaluate the right-hand side together; kernel 4 runs one wavefront
+// per 64 rays and (step, stage, field).
+// Lanes of adj_forward_kernel per ray.  Each lane interpolates 4 / lanes of
+// the ray's fields, and every lane evaluates the right-hand side, so a
+// wavefront runs the right-hand side once for its 64 / lanes rays.  Two lanes
+// per ray run the 128^3 / Nside 64 / 64-step gradient in 12.3 ms against 13.3 ms
+// with four (which leave three wavefronts per SIMD for a two-wavefront
+// occupancy, and spill), and 397 against 431 ms at 256^3 / Nside 256 / 128
+// steps; one lane per ray runs one wavefront per SIMD and is slower still.
+constexpr int kAdjLanesPerRay = 2;
+constexpr int kAdjFieldsPerLane = 4 / kAdjLanesPerRay;
+constexpr int kAdjRays = 64 / kAdjLanesPerRay;  // rays per 64-lane workgroup of adj_forward_kernel
+static_assert(kAdjLanesPerRay == 4 || kAdjLanesPerRay == 2, "a ray's lanes split its four fields evenly");
 constexpr int kAdjRec = 19 + 4 * 20;            // doubles per stage record: state, then 20 per field
 constexpr int64_t kAdjChunkBytes = int64_t(8) << 30;  // per-chunk buffers (records, checkpoints, dL/dy)
 
@@ -688,19 +698,21 @@ __device__ __forceinline__ bool adj_ray(const AdjArgs& A, int64_t l, int64_t& sl
 // state, storing checkpoints, maps and the final dL/dy, and the stage records
 // of the steps from A.rec_from on: the last segment's, which the reverse
 // sweep needs first) or one segment from its checkpoint (kRecord = true:
-// storing its stage records).  Lane 4 r + f of the
-// workgroup serves ray r and field f.  The ray's state, its two stage states
-// and the 40 interpolants live in LDS; component j of a state is updated by
-// lane j mod 4 of the ray (each lane evaluates the whole right-hand side).
+// storing its stage records).  Lane kAdjLanesPerRay r + h of the workgroup
+// serves ray r and interpolates fields kAdjFieldsPerLane h .. (potential and
+// B_x, then B_y and B_z, with two lanes per ray).  The ray's state, its two
+// stage states and the 40 interpolants live in LDS; component j of a state is
+// updated by lane j mod kAdjLanesPerRay of the ray (each lane evaluates the
+// whole right-hand side).
 template <bool kRecord>
 __global__ __launch_bounds__(64, 2) void adj_forward_kernel(AdjArgs A) {
     __shared__ double sy[19][kAdjRays], sbuf[2][19][kAdjRays], sq[40][kAdjRays];
-    const int r = threadIdx.x >> 2, f = threadIdx.x & 3;
+    const int r = threadIdx.x / kAdjLanesPerRay, h = threadIdx.x % kAdjLanesPerRay;
     const int64_t l = (int64_t)blockIdx.x * kAdjRays + r;
-    if (l >= A.count) return;  // the four lanes of a ray leave together
+    if (l >= A.count) return;  // the lanes of a ray leave together
     int64_t slot, pix;
     if (!adj_ray(A, l, slot, pix)) {
-        if (!kRecord && A.maps64 && f == 0)
+        if (!kRecord && A.maps64 && h == 0)
             for (int m = 0; m < 6; ++m) A.maps64[m * A.total + slot] = __builtin_nan("");
         return;
     }
@@ -712,39 +724,44 @@ __global__ __launch_bounds__(64, 2) void adj_forward_kernel(AdjArgs A) {
         sincos(ph, &sp, &cp);
         const double init[19] = {A.obs[0], A.obs[1], A.obs[2], -s * cp, -s * sp, -z, z * cp, z * sp, -s, sp, -cp,
                                  0.0,      0.0,      -1.0,     0.0,      0.0,     0.0, 0.0,    0.0};
-        if (f == 0)
+        if (h == 0)
             for (int j = 0; j < 19; ++j) sy[j][r] = init[j];
     } else {
         const double* ck = A.ckpt + (int64_t)A.seg * 19 * C + l;
-        for (int j = f; j < 19; j += 4) sy[j][r] = ck[j * C];
+        for (int j = h; j < 19; j += kAdjLanesPerRay) sy[j][r] = ck[j * C];
     }
     __syncthreads();
     const size_t py = (size_t)A.G.pad, pz = (size_t)A.G.pad * (size_t)A.G.pad;
-    const double* grid = srt::adj::field_grid64(A.G, f);
     const int steps = kRecord ? A.seg_steps : A.n_steps;
-    double acc[5];  // this lane's components of the RK4 stage sum
+    constexpr int kComps = (19 + kAdjLanesPerRay - 1) / kAdjLanesPerRay;  // state components per lane
+    double acc[kComps];  // this lane's components of the RK4 stage sum
     for (int step = 0; step < steps; ++step) {
         if (!kRecord && step % K == 0) {
             double* ck = A.ckpt + (int64_t)(step / K) * 19 * C + l;
-            for (int j = f; j < 19; j += 4) ck[j * C] = sy[j][r];
+            for (int j = h; j < 19; j += kAdjLanesPerRay) ck[j * C] = sy[j][r];
         }
         for (int stage = 0; stage < 4; ++stage) {
             // stage state: y, then y + k1 / 2 in sbuf[0], y + k2 / 2 in sbuf[1], y + k3 in sbuf[0]
             double(*S)[kAdjRays] = stage == 0 ? sy : sbuf[(stage - 1) & 1];
             const double x[3] = {S[0][r], S[1][r], S[2][r]};
-            const srt::adj::Stencil st = srt::adj::field_stencil(A.G, x, f);
-            double o[20];
             const bool record = kRecord || step >= A.rec_from;
+            double* const rq = record ? A.rec + (int64_t)((kRecord ? step : step - A.rec_from) * 4 + stage) *
+                                                    kAdjRec * C + l
+                                      : nullptr;
             if (record)
-                srt::adj::field_interp<true>(grid, py, pz, st, o);
-            else
-                srt::adj::field_interp<false>(grid, py, pz, st, o);
-            for (int k = 0; k < 10; ++k) sq[10 * f + k][r] = o[k];
-            if (record) {
-                const int rs = kRecord ? step : step - A.rec_from;
-                double* rq = A.rec + (int64_t)(rs * 4 + stage) * kAdjRec * C + l;
-                for (int j = f; j < 19; j += 4) rq[j * C] = S[j][r];
-                for (int k = 0; k < 20; ++k) rq[(19 + 20 * f + k) * C] = o[k];
+                for (int j = h; j < 19; j += kAdjLanesPerRay) rq[j * C] = S[j][r];
+            for (int ff = 0; ff < kAdjFieldsPerLane; ++ff) {
+                const int f = h * kAdjFieldsPerLane + ff;
+                const srt::adj::Stencil st = srt::adj::field_stencil(A.G, x, f);
+                const double* grid = srt::adj::field_grid64(A.G, f);
+                double o[20];
+                if (record)
+                    srt::adj::field_interp<true>(grid, py, pz, st, o);
+                else
+                    srt::adj::field_interp<false>(grid, py, pz, st, o);
+                for (int k = 0; k < 10; ++k) sq[10 * f + k][r] = o[k];
+                if (record)
+                    for (int k = 0; k < 20; ++k) rq[(19 + 20 * f + k) * C] = o[k];
             }
             __syncthreads();  // every field's interpolants are in sq
             double sv[19], dy[19];
@@ -752,11 +769,15 @@ __global__ __launch_bounds__(64, 2) void adj_forward_kernel(AdjArgs A) {
             srt::adj::RhsTape T;
             srt::adj::rhs_forward_from(sv, &sq[0][r], kAdjRays, T, dy);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const int j = f + 4 * m;
+            for (int m = 0; m < kComps; ++m) {
+                const int j = h + kAdjLanesPerRay * m;
                 if (j >= 19) break;
                 // dy[j] by constant indices (a runtime index would put dy in scratch)
-                const double d = f == 0 ? dy[4 * m] : (f == 1 ? dy[4 * m + 1] : (f == 2 ? dy[4 * m + 2] : dy[(4 * m + 3) % 19]));
+                double d;
+                if constexpr (kAdjLanesPerRay == 4)
+                    d = h == 0 ? dy[4 * m] : (h == 1 ? dy[4 * m + 1] : (h == 2 ? dy[4 * m + 2] : dy[(4 * m + 3) % 19]));
+                else
+                    d = h == 0 ? dy[2 * m] : dy[(2 * m + 1) % 19];
                 if (stage == 0)
                     acc[m] = d;
                 else if (stage < 3)
@@ -773,9 +794,9 @@ __global__ __launch_bounds__(64, 2) void adj_forward_kernel(AdjArgs A) {
     {
         const int nseg = (A.n_steps + K - 1) / K;
         double* ck = A.ckpt + (int64_t)nseg * 19 * C + l;
-        for (int j = f; j < 19; j += 4) ck[j * C] = sy[j][r];
+        for (int j = h; j < 19; j += kAdjLanesPerRay) ck[j * C] = sy[j][r];
     }
-    if (f != 0) return;
+    if (h != 0) return;
     double y[19];
     for (int j = 0; j < 19; ++j) y[j] = sy[j][r];
     // maps (.cu:566-582) in FP64, and dL/dy at the end from dL/dmaps
