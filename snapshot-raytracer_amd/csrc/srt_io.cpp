@@ -123,8 +123,9 @@ bool parallel_pread(const std::string& file, uint64_t offset, void* dst, uint64_
 
 // File offset of a dataset whose stored bytes are exactly the floats the
 // reader returns (per_voxel = 1: little-endian IEEE binary32 elements; 3: a
-// one-dimensional array of 3 of them): contiguous layout, no filters, every
-// element written, no user block (so the offset is from the start of the file).
+// one-dimensional array of 3 of them): contiguous layout in the file itself (no
+// filters, no external storage), every element written, no user block (so the
+// offset is from the start of the file).
 // 0 when the dataset is stored any other way.
 uint64_t raw_float_offset(hid_t file, hid_t dset, hid_t ftype, int per_voxel, uint64_t n_floats) {
     if (H5Tequal(H5T_NATIVE_FLOAT, H5T_IEEE_F32LE) <= 0) return 0;
@@ -136,7 +137,9 @@ uint64_t raw_float_offset(hid_t file, hid_t dset, hid_t ftype, int per_voxel, ui
         if (base.id < 0 || H5Tequal(base.id, H5T_IEEE_F32LE) <= 0) return 0;
     }
     H5Handle dcpl(H5Dget_create_plist(dset), H5Pclose);
-    if (dcpl.id < 0 || H5Pget_layout(dcpl.id) != H5D_CONTIGUOUS || H5Pget_nfilters(dcpl.id) != 0) return 0;
+    if (dcpl.id < 0 || H5Pget_layout(dcpl.id) != H5D_CONTIGUOUS || H5Pget_nfilters(dcpl.id) != 0 ||
+        H5Pget_external_count(dcpl.id) != 0)
+        return 0;
     H5Handle fcpl(H5Fget_create_plist(file), H5Pclose);
     hsize_t userblock = 1;
     if (fcpl.id < 0 || H5Pget_userblock(fcpl.id, &userblock) < 0 || userblock != 0) return 0;

@@ -65,8 +65,9 @@ def test_hdf5_round_trip(sio, iolib, tmp_path):
     assert np.array_equal(got, (phi.astype(np.float64) * 1.000000001).astype(np.float32))
 
 
-def write_h5(sio, path, arr, file_type, mem_type, chunk=None, userblock=0):
-    """One rank-3 dataset /data with optional chunked layout and user block, through the HDF5 C library."""
+def write_h5(sio, path, arr, file_type, mem_type, chunk=None, userblock=0, external=None):
+    """One rank-3 dataset /data with optional chunked layout, user block or external raw-data file, through the
+    HDF5 C library."""
     L = sio._lib()
     hid = ctypes.c_int64
     L.H5Pcreate.restype = hid
@@ -74,13 +75,17 @@ def write_h5(sio, path, arr, file_type, mem_type, chunk=None, userblock=0):
     L.H5Pset_chunk.argtypes = [hid, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
     L.H5Pset_userblock.argtypes = [hid, ctypes.c_uint64]
     L.H5Pclose.argtypes = [hid]
+    L.H5Pset_external.argtypes = [hid, ctypes.c_char_p, ctypes.c_int64, ctypes.c_uint64]
     fcpl = dcpl = 0
     if userblock:
         fcpl = L.H5Pcreate(sio._global("H5P_CLS_FILE_CREATE_ID_g"))
         assert L.H5Pset_userblock(fcpl, userblock) >= 0
-    if chunk:
+    if chunk or external:
         dcpl = L.H5Pcreate(sio._global("H5P_CLS_DATASET_CREATE_ID_g"))
+    if chunk:
         assert L.H5Pset_chunk(dcpl, 3, (ctypes.c_uint64 * 3)(*chunk)) >= 0
+    if external:
+        assert L.H5Pset_external(dcpl, external.encode(), 0, arr.nbytes) >= 0
     fid = L.H5Fcreate(path.encode(), 0x0002, fcpl, 0)
     assert fid >= 0
     sid = L.H5Screate_simple(3, (ctypes.c_uint64 * 3)(*arr.shape[:3]), None)
@@ -97,8 +102,8 @@ def write_h5(sio, path, arr, file_type, mem_type, chunk=None, userblock=0):
 
 def test_hdf5_raw_and_converted_reads(sio, iolib, tmp_path):
     """Contiguous little-endian float datasets are read straight from the file by several threads (160^3: a
-    16 MB potential, a 49 MB shift); big-endian floats, chunked datasets and files with a user block go through
-    H5Dread.  Every way returns the same floats."""
+    16 MB potential, a 49 MB shift); big-endian floats, chunked datasets, files with a user block and data in an
+    external raw file go through H5Dread.  Every way returns the same floats."""
     rng = np.random.default_rng(5)
     phi = rng.standard_normal((160, 160, 160)).astype(np.float32)
     shift = rng.standard_normal((160, 160, 160, 3)).astype(np.float32)
@@ -119,6 +124,10 @@ def test_hdf5_raw_and_converted_reads(sio, iolib, tmp_path):
              ("ub3.h5", small3, arr_native, arr_native, None, 1024, 3)]
     for name, arr, ft, mt, chunk, ub, comps in cases:
         write_h5(sio, str(tmp_path / name), arr, ft, mt, chunk, ub)
+    for name, arr, t, comps in (("ext.h5", small, native, 1), ("ext3.h5", small3, arr_native, 3)):
+        write_h5(sio, str(tmp_path / name), arr, t, t, external=str(tmp_path / (name + ".raw")))
+        cases.append((name, arr, t, t, None, 0, comps))
+    for name, arr, ft, mt, chunk, ub, comps in cases:
         fn = "srtio_read_scalar_field" if comps == 1 else "srtio_read_vector_field"
         got, e = read_field(iolib, fn, str(tmp_path / name), comps)
         assert e == "" and np.array_equal(got, arr), name
