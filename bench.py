@@ -129,13 +129,15 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0,
                    help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every CPU this process "
                         "may run on)")
-    p.add_argument("--workload", default="headline", choices=["headline", "large", "adjoint", "batch", "layouts", "cli"],
+    p.add_argument("--workload", default="headline",
+                   choices=["headline", "large", "adjoint", "batch", "layouts", "cli", "max"],
                    help="headline: 512^3 / Nside 1024 / 256 steps (the benchmark's JSON line); "
                         "large: 1024^3 / Nside 2048 / 512 steps, one full-sky launch; adjoint: 128^3 / "
                         "Nside 64 / 64 steps gradient with finite-difference check; batch: 48 uneven "
                         "instances in one plan launch (persistent and dispatch schedules); layouts: the headline workload's mixed "
                         "kernel on the padded cubes and on bicubic records of several brick edges; cli: the "
-                        "command-line drop-in end to end on the headline workload (HDF5 files in, FITS maps out)")
+                        "command-line drop-in end to end on the headline workload (HDF5 files in, FITS maps out); "
+                        "max: the largest grid, 2048^3 / Nside 2048 / 512 steps, on the padded cubes")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first run one device copy of 1 GiB (a known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -260,6 +262,50 @@ def workload_large(args, pkg, fields, dev):
     print(json.dumps({"workload": "large", "config": {"grid": n, "nside": nside, "n_steps": n_steps,
                                                       "rays": rays, "precision": "mixed"}, "results": res,
                       "roofline": roof}))
+
+
+def workload_max(args, pkg, fields, dev):
+    """The largest grid the library accepts: 2048^3 (padded cubes of 2051^3, 138 GB of device memory), a field of
+    period 128 cells (fields.periodic_field: separable modes at the other workloads' amplitudes; the spectral
+    generator would need more memory than the card has beside the grids), Nside 2048 full sky, 512 steps, NESTED
+    lanes.  Neither the bicubic records (4.4 TB) nor the FP64 grid copies (276 GB beside the cubes) fit, so the
+    mixed kernel reads the padded cubes; the REF kernel is timed too."""
+    import torch
+    n, nside, n_steps = 2048, 2048, 512
+    rays = 12 * nside * nside
+    torch.cuda.empty_cache()
+    free, total = torch.cuda.mem_get_info()
+    if free < 290e9:
+        print(json.dumps({"workload": "max", "skipped": f"needs ~280 GB of free device memory, {free / 1e9:.0f} GB free"}))
+        return
+    t0 = time.time()
+    tr = pkg.Tracer(0)
+    phi, shift = fields.periodic_field(n, 128, device=dev)
+    tr.upload(phi, shift)
+    del phi, shift
+    torch.cuda.empty_cache()
+    free, total = torch.cuda.mem_get_info()
+    print(f"[bench] max: field + upload {time.time() - t0:.1f} s; device memory free {free / 1e9:.1f} of "
+          f"{total / 1e9:.1f} GB", file=sys.stderr, flush=True)
+    out = torch.empty(6 * rays, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {"device_memory_free_gb": free / 1e9, "device_memory_total_gb": total / 1e9}
+    for name, prec in (("mixed_taps", pkg.PREC_MIXED), ("ref", pkg.PREC_REF)):
+        tr.trace_device(nside, n_steps, out, precision=prec, order=pkg.ORDER_NESTED, stream=stream)
+        ms = tr.last_trace_ms()
+        finite = bool(torch.isfinite(out).all())
+        res[name] = {"kernel_ms": ms, "ray_steps_per_s": rays * n_steps / (ms / 1000.0), "maps_finite": finite}
+        print(f"[bench] max {name}: {ms:.0f} ms, {res[name]['ray_steps_per_s']:.3e} ray-steps/s",
+              file=sys.stderr, flush=True)
+    ms = res["mixed_taps"]["kernel_ms"]
+    ach = ALGO_FLOPS_PER_RAY_STEP * rays * n_steps / (ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
+            "frac_kernel_time_only": ach / PEAK_TFLOPS["mixed"], "algorithmic_flops_per_ray_step": ALGO_FLOPS_PER_RAY_STEP,
+            "kernel": "trace_kernel<mixed, padded cubes>, nested lanes"}
+    print(json.dumps({"workload": "max", "config": {"grid": n, "nside": nside, "n_steps": n_steps, "rays": rays,
+                                                    "field": "period 128 cells (fields.periodic_field)"},
+                      "results": res, "roofline": roof}))
+    tr.close()
 
 
 def workload_adjoint(args, pkg, fields, dev):
@@ -572,7 +618,7 @@ def main():
         del src, dst
     if args.workload != "headline":
         {"large": workload_large, "adjoint": workload_adjoint, "batch": workload_batch,
-         "layouts": workload_layouts, "cli": workload_cli}[args.workload](
+         "layouts": workload_layouts, "cli": workload_cli, "max": workload_max}[args.workload](
             args, pkg, fields, dev)
         return
     n, nside, n_steps = args.grid, args.nside, args.n_steps

@@ -99,6 +99,28 @@ def snapshot(n: int, seed: int, phi_rms: float = 1e-5, shift_rms: float = 1e-7, 
     return phi, shift
 
 
+def periodic_field(n: int, period: int = 128, device="cpu"):
+    """A field of period `period` cells along every axis, for grids too large to generate spectrally:
+    phi (n, n, n) a sum of one-dimensional modes along z, y and x (amplitudes 1e-5, 0.7e-5, 0.5e-5), and the
+    interleaved shift (n, n, n, 3) with B_x varying along y, B_y along z and B_z along x (amplitudes 1e-7,
+    0.8e-7, 0.6e-7), float32, arrays (z, y, x) with x fastest.  An n = k * period grid is the period^3 grid
+    tiled k^3 times with bit-identical values."""
+    t = torch.arange(period, dtype=torch.float64, device=device) * (2 * math.pi / period)
+    ax = (1e-5 * torch.sin(t)).float()
+    ay = (0.7e-5 * torch.cos(2 * t + 0.4)).float()
+    az = (0.5e-5 * torch.sin(3 * t + 1.1)).float()
+    bx = (1e-7 * torch.cos(t + 0.2)).float()
+    by = (0.8e-7 * torch.sin(2 * t)).float()
+    bz = (0.6e-7 * torch.cos(t + 2.0)).float()
+    idx = torch.arange(n, device=device) % period
+    phi = az[idx].view(n, 1, 1) + ay[idx].view(1, n, 1) + ax[idx].view(1, 1, n)
+    shift = torch.empty((n, n, n, 3), dtype=torch.float32, device=device)
+    shift[..., 0] = by[idx].view(1, n, 1)
+    shift[..., 1] = bz[idx].view(n, 1, 1)
+    shift[..., 2] = bx[idx].view(1, 1, n)
+    return phi, shift
+
+
 def interleave_shift(shift: torch.Tensor) -> torch.Tensor:
     """(3, n, n, n) planar -> (n, n, n, 3) interleaved, the HDF5 H5T_ARRAY{3} layout."""
     return shift.permute(1, 2, 3, 0).contiguous()

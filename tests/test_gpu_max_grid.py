@@ -17,8 +17,6 @@ At this size the bicubic records (4.4 TB) and the FP64 grid copies do not
 fit: setting the record layout and an FP64 trace fail with
 SRT_ERR_OUT_OF_MEMORY, and the context keeps tracing from the padded cubes.
 """
-import math
-
 import numpy as np
 import pytest
 
@@ -30,36 +28,16 @@ OBS_BIG = OBS_SMALL + 15 * PERIOD  # 2020.37, 2019.61, 2030.29: 48 steps reach p
 ERR_OUT_OF_MEMORY = 3
 
 
-def periodic_field(n, dev):
-    """phi (n, n, n) and interleaved shift (n, n, n, 3), float32, period 128 cells."""
-    import torch
-    t = torch.arange(PERIOD, dtype=torch.float64, device=dev) * (2 * math.pi / PERIOD)
-    ax = (1e-5 * torch.sin(t)).float()
-    ay = (0.7e-5 * torch.cos(2 * t + 0.4)).float()
-    az = (0.5e-5 * torch.sin(3 * t + 1.1)).float()
-    bx = (1e-7 * torch.cos(t + 0.2)).float()
-    by = (0.8e-7 * torch.sin(2 * t)).float()
-    bz = (0.6e-7 * torch.cos(t + 2.0)).float()
-    idx = torch.arange(n, device=dev) % PERIOD
-    # arrays are (z, y, x) with x fastest
-    phi = az[idx].view(n, 1, 1) + ay[idx].view(1, n, 1) + ax[idx].view(1, 1, n)
-    shift = torch.empty((n, n, n, 3), dtype=torch.float32, device=dev)
-    shift[..., 0] = by[idx].view(1, n, 1)   # B_x varies along y
-    shift[..., 1] = bz[idx].view(n, 1, 1)   # B_y along z
-    shift[..., 2] = bx[idx].view(1, 1, n)   # B_z along x
-    return phi, shift
-
-
 @pytest.fixture(scope="module")
-def small(srt):
+def small(srt, fields):
     tr = srt.Tracer(0)
-    phi, shift = periodic_field(PERIOD, "cuda:0")
+    phi, shift = fields.periodic_field(PERIOD, PERIOD, "cuda:0")
     tr.upload(phi, shift)
     yield tr
     tr.close()
 
 
-def test_max_grid_matches_tiled_small_grid(srt, small):
+def test_max_grid_matches_tiled_small_grid(srt, small, fields):
     import torch
     torch.cuda.empty_cache()
     free, total = torch.cuda.mem_get_info()
@@ -67,7 +45,7 @@ def test_max_grid_matches_tiled_small_grid(srt, small):
         pytest.skip(f"needs ~280 GB of free device memory, {free / 1e9:.0f} GB free")
     print(f"[gpu] {BIG}^3 periodic field: generate, upload", flush=True)
     big = srt.Tracer(0)
-    phi, shift = periodic_field(BIG, "cuda:0")
+    phi, shift = fields.periodic_field(BIG, PERIOD, "cuda:0")
     big.upload(phi, shift)
     del phi, shift
     torch.cuda.empty_cache()
