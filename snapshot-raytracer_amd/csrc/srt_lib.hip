@@ -1836,16 +1836,33 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
 }  // extern "C"
 
 // ------------------------------------------------------------ batch plans ----
+// One kernel launch of a plan: the items traced by the same kernel, with that
+// launch's device tables.  A plan has one launch, or two when it is a mixed
+// plan whose contexts differ in layout: the items whose contexts hold bicubic
+// records run the record kernel and the others the padded-cube kernel, each as
+// its own srt_trace would, so every item's maps equal its srt_trace's.
+struct PlanLaunch {
+    std::vector<int> items;                   // indices into srt_plan::items
+    bool bicubic = false;                     // mixed launch over bicubic records
+    bool pow2 = false;                        // every item's grid edge is a power of two
+    BatchInst* d_inst = nullptr;              // the items' table
+    unsigned long long* d_counter = nullptr;  // kMaxTileGroups tile counters (persistent schedule)
+    int64_t* d_parts = nullptr;  // batch_dispatch_kernel's table: n + kMaxTileGroups * (n + 1), n items
+    int dispatch_lanes = 0;      // lanes per dispatch tile the range table was built for
+    int64_t dispatch_blocks = 0;  // workgroups of a dispatch launch
+    int64_t total_tiles = 0;      // tiles of the persistent schedule
+};
+
+constexpr int kMaxPlanLaunches = 2;
+
 struct srt_plan {
     int device = 0;
     int precision = SRT_PREC_MIXED;
-    bool bicubic = false;  // mixed plan whose contexts all hold bicubic records
-    bool pow2 = false;     // every item's grid edge is a power of two
     int n_inst = 0;
     int64_t total_rays = 0, total_ray_steps = 0;
     std::vector<srt_batch_item> items;
-    BatchInst* d_inst = nullptr;
-    unsigned long long* d_counter = nullptr;  // kMaxTileGroups tile counters
+    PlanLaunch launches[kMaxPlanLaunches];
+    int n_launches = 0;
     int groups = 8;                            // tile ranges (see batch_kernel)
     int order = SRT_ORDER_NESTED;              // lane order within each item
     void* order_buf = nullptr;                 // per-item lane -> slot maps + sort space
@@ -1855,12 +1872,8 @@ struct srt_plan {
     // dispatch by default: at 1024^3 / Nside 2048 / 512 steps a one-item plan takes 3.65 s against 3.77 s
     // persistent (the single trace launch: 3.66 s), and 48 uneven snapshots 68.6 against 86.7 ms
     int schedule = SRT_SCHEDULE_DISPATCH;
-    int64_t* d_parts = nullptr;  // batch_dispatch_kernel's table: n_inst + kMaxTileGroups * (n_inst + 1)
-    int dispatch_lanes = 0;      // lanes per dispatch tile the range table was built for
-    int64_t dispatch_blocks = 0;  // workgroups of a dispatch launch
     int tile_rays = 64;  // one 64-ray group per claim: in-flight rays of an XCD stay as compact as a trace launch's
     int blocks = 0;
-    int64_t total_tiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
@@ -1912,74 +1925,88 @@ int plan_build_orders(srt_plan* p) {
     return SRT_OK;
 }
 
-// (re)builds the device instance table for the plan's tile size from the
+// (re)builds the launches' device tables for the plan's tile size from the
 // contexts' current buffers
 int plan_build_table(srt_plan* p) {
     for (int i = 0; i < p->n_inst; ++i)
         if (!p->items[i].ctx->grids)
             return fail(SRT_ERR_NOT_UPLOADED, "plan item " + std::to_string(i) + ": context without snapshot");
-    std::vector<BatchInst> h(p->n_inst);
-    int64_t tiles = 0;
-    p->bicubic = p->precision == SRT_PREC_MIXED;
-    for (int i = 0; i < p->n_inst; ++i) p->bicubic = p->bicubic && p->items[i].ctx->rec != nullptr;
-    p->pow2 = true;
-    for (int i = 0; i < p->n_inst; ++i) p->pow2 = p->pow2 && (p->items[i].ctx->n & (p->items[i].ctx->n - 1)) == 0;
-    for (int i = 0; i < p->n_inst; ++i) {
-        const srt_batch_item& it = p->items[i];
-        srt_context* c = it.ctx;
-        BatchInst& b = h[i];
-        const size_t pvox = (size_t)c->pad * c->pad * c->pad;
-        b.G.phi = c->grids;
-        for (int a = 0; a < 3; ++a) b.G.b[a] = c->grids + (size_t)(1 + a) * pvox;
-        b.G.n = c->n;
-        b.G.pad = c->pad;
-        b.G.inv_n = 1.0 / (double)c->n;
-        if (p->precision == SRT_PREC_FP64) {
-            const int rc = ensure_grids64(c, nullptr);
-            if (rc != SRT_OK) return rc;
+    // the launches: a mixed plan's items whose contexts hold records, then the others
+    std::vector<int> with_rec, rest;
+    for (int i = 0; i < p->n_inst; ++i)
+        (p->precision == SRT_PREC_MIXED && p->items[i].ctx->rec ? with_rec : rest).push_back(i);
+    p->n_launches = 0;
+    for (std::vector<int>* members : {&with_rec, &rest})
+        if (!members->empty()) {
+            PlanLaunch& L = p->launches[p->n_launches++];
+            L.items = *members;
+            L.bicubic = members == &with_rec;
         }
-        set_grids64(c, b.G);
-        set_records(c, b.G);
-        b.nside = it.nside;
-        b.pix_begin = it.pix_begin;
-        b.count = it.count;
-        b.n_steps = it.n_steps;
-        for (int k = 0; k < 3; ++k) b.obs[k] = it.observer_set ? it.observer[k] : 0.5 * (double)c->n;
-        b.out = it.out;
-        b.lane_slot = p->item_map.empty() ? nullptr : p->item_map[i];
-        b.tile_begin = tiles;
-        const int64_t per_tile = (int64_t)p->tile_rays * (kBlock / 64);
-        tiles += (it.count + per_tile - 1) / per_tile;
-    }
-    p->total_tiles = tiles;
-    // range table of the dispatch schedule (batch_dispatch_kernel), for the
-    // workgroup size of the form it will launch
-    const int lanes = p->bicubic ? 64 : kBlock;
-    const int n = p->n_inst, g = p->groups;
-    std::vector<int64_t> parts((size_t)n + (size_t)g * (n + 1));
-    for (int i = 0; i < n; ++i) parts[i] = i;
-    std::stable_sort(parts.begin(), parts.begin() + n,
-                     [&](int64_t a, int64_t b) { return p->items[a].n_steps > p->items[b].n_steps; });
-    int64_t longest = 0;
-    for (int x = 0; x < g; ++x) {
-        int64_t* pb = parts.data() + n + (size_t)x * (n + 1);
-        int64_t acc = 0;
-        for (int j = 0; j < n; ++j) {
-            pb[j] = acc;
-            const int64_t t = (p->items[parts[j]].count + lanes - 1) / lanes;
-            acc += t * (x + 1) / g - t * x / g;
-        }
-        pb[n] = acc;
-        longest = std::max(longest, acc);
-    }
-    if (longest * g * lanes >= (int64_t(1) << 32))
-        return fail(SRT_ERR_UNSUPPORTED, "plan: the dispatch schedule's launch would hold 2^32 or more lanes");
-    p->dispatch_lanes = lanes;
-    p->dispatch_blocks = longest * g;
     SRT_HIP(hipSetDevice(p->device));
     SRT_HIP(hipDeviceSynchronize());
-    SRT_HIP(hipMemcpy(p->d_inst, h.data(), h.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
-    SRT_HIP(hipMemcpy(p->d_parts, parts.data(), parts.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    for (int li = 0; li < p->n_launches; ++li) {
+        PlanLaunch& L = p->launches[li];
+        const int n = (int)L.items.size(), g = p->groups;
+        L.pow2 = true;
+        for (int idx : L.items) L.pow2 = L.pow2 && (p->items[idx].ctx->n & (p->items[idx].ctx->n - 1)) == 0;
+        std::vector<BatchInst> h(n);
+        int64_t tiles = 0;
+        for (int j = 0; j < n; ++j) {
+            const int i = L.items[j];
+            const srt_batch_item& it = p->items[i];
+            srt_context* c = it.ctx;
+            BatchInst& b = h[j];
+            const size_t pvox = (size_t)c->pad * c->pad * c->pad;
+            b.G.phi = c->grids;
+            for (int a = 0; a < 3; ++a) b.G.b[a] = c->grids + (size_t)(1 + a) * pvox;
+            b.G.n = c->n;
+            b.G.pad = c->pad;
+            b.G.inv_n = 1.0 / (double)c->n;
+            if (p->precision == SRT_PREC_FP64) {
+                const int rc = ensure_grids64(c, nullptr);
+                if (rc != SRT_OK) return rc;
+            }
+            set_grids64(c, b.G);
+            set_records(c, b.G);
+            b.nside = it.nside;
+            b.pix_begin = it.pix_begin;
+            b.count = it.count;
+            b.n_steps = it.n_steps;
+            for (int k = 0; k < 3; ++k) b.obs[k] = it.observer_set ? it.observer[k] : 0.5 * (double)c->n;
+            b.out = it.out;
+            b.lane_slot = p->item_map.empty() ? nullptr : p->item_map[i];
+            b.tile_begin = tiles;
+            const int64_t per_tile = (int64_t)p->tile_rays * (kBlock / 64);
+            tiles += (it.count + per_tile - 1) / per_tile;
+        }
+        L.total_tiles = tiles;
+        // range table of the dispatch schedule (batch_dispatch_kernel), for the
+        // workgroup size of the form this launch runs
+        const int lanes = L.bicubic ? 64 : kBlock;
+        std::vector<int64_t> parts((size_t)n + (size_t)g * (n + 1));
+        for (int j = 0; j < n; ++j) parts[j] = j;
+        std::stable_sort(parts.begin(), parts.begin() + n, [&](int64_t a, int64_t b) {
+            return p->items[L.items[a]].n_steps > p->items[L.items[b]].n_steps;
+        });
+        int64_t longest = 0;
+        for (int x = 0; x < g; ++x) {
+            int64_t* pb = parts.data() + n + (size_t)x * (n + 1);
+            int64_t acc = 0;
+            for (int j = 0; j < n; ++j) {
+                pb[j] = acc;
+                const int64_t t = (p->items[L.items[parts[j]]].count + lanes - 1) / lanes;
+                acc += t * (x + 1) / g - t * x / g;
+            }
+            pb[n] = acc;
+            longest = std::max(longest, acc);
+        }
+        if (longest * g * lanes >= (int64_t(1) << 32))
+            return fail(SRT_ERR_UNSUPPORTED, "plan: the dispatch schedule's launch would hold 2^32 or more lanes");
+        L.dispatch_lanes = lanes;
+        L.dispatch_blocks = longest * g;
+        SRT_HIP(hipMemcpy(L.d_inst, h.data(), h.size() * sizeof(BatchInst), hipMemcpyHostToDevice));
+        SRT_HIP(hipMemcpy(L.d_parts, parts.data(), parts.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
     p->item_gen.resize(p->n_inst);
     for (int i = 0; i < p->n_inst; ++i) p->item_gen[i] = p->items[i].ctx->generation;
     return SRT_OK;
@@ -1996,50 +2023,51 @@ int plan_refresh(srt_plan* p) {
     return plan_build_table(p);
 }
 
-// batch_kernel<F> for plan p, specialised when every item's grid edge is a power of two
+// batch_kernel<F> for launch L of plan p, specialised when every item's grid edge is a power of two
 template <class F>
-void launch_batch(const srt_plan* p, hipStream_t st) {
-    if (p->pow2)
+void launch_batch(const srt_plan* p, const PlanLaunch& L, hipStream_t st) {
+    const int n = (int)L.items.size();
+    if (L.pow2)
         hipLaunchKernelGGL(batch_kernel<srt::Pow2Grid<F>>, dim3(p->blocks), dim3(kBlock), lds_bytes<F>(), st,
-                           p->d_inst, p->n_inst, p->total_tiles, p->tile_rays, p->groups, p->d_counter);
+                           L.d_inst, n, L.total_tiles, p->tile_rays, p->groups, L.d_counter);
     else
-        hipLaunchKernelGGL(batch_kernel<F>, dim3(p->blocks), dim3(kBlock), lds_bytes<F>(), st, p->d_inst, p->n_inst,
-                           p->total_tiles, p->tile_rays, p->groups, p->d_counter);
+        hipLaunchKernelGGL(batch_kernel<F>, dim3(p->blocks), dim3(kBlock), lds_bytes<F>(), st, L.d_inst, n,
+                           L.total_tiles, p->tile_rays, p->groups, L.d_counter);
 }
 
-// batch_dispatch_kernel<F> for plan p (the dispatch schedule)
+// batch_dispatch_kernel<F> for launch L of plan p (the dispatch schedule)
 template <class F>
-int launch_batch_dispatch(const srt_plan* p, hipStream_t st) {
+int launch_batch_dispatch(const srt_plan* p, const PlanLaunch& L, hipStream_t st) {
     constexpr int lanes = trace_lanes<F>();
     constexpr unsigned lds = lds_bytes<F, lanes>();
-    if (lanes != p->dispatch_lanes)
+    if (lanes != L.dispatch_lanes)
         return fail(SRT_ERR_DEVICE, "plan: range table built for another workgroup size");
-    if (p->dispatch_blocks == 0) {  // no rays: keep the launch sequence non-empty for graph capture
-        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, p->d_counter, p->groups);
+    if (L.dispatch_blocks == 0) {  // no rays: keep the launch sequence non-empty for graph capture
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, L.d_counter, p->groups);
         return SRT_OK;
     }
-    const dim3 grid((unsigned)p->dispatch_blocks);
-    if (p->pow2)
-        hipLaunchKernelGGL(batch_dispatch_kernel<srt::Pow2Grid<F>>, grid, dim3(lanes), lds, st, p->d_inst, p->n_inst,
-                           p->d_parts, p->groups);
+    const dim3 grid((unsigned)L.dispatch_blocks);
+    const int n = (int)L.items.size();
+    if (L.pow2)
+        hipLaunchKernelGGL(batch_dispatch_kernel<srt::Pow2Grid<F>>, grid, dim3(lanes), lds, st, L.d_inst, n,
+                           L.d_parts, p->groups);
     else
-        hipLaunchKernelGGL(batch_dispatch_kernel<F>, grid, dim3(lanes), lds, st, p->d_inst, p->n_inst, p->d_parts,
-                           p->groups);
+        hipLaunchKernelGGL(batch_dispatch_kernel<F>, grid, dim3(lanes), lds, st, L.d_inst, n, L.d_parts, p->groups);
     return SRT_OK;
 }
 
-int plan_enqueue_dispatch(srt_plan* p, hipStream_t st) {
+int plan_enqueue_dispatch(srt_plan* p, const PlanLaunch& L, hipStream_t st) {
     int rc;
     switch (p->precision) {
         case SRT_PREC_FP64:
-            rc = launch_batch_dispatch<srt::FormFP64>(p, st);
+            rc = launch_batch_dispatch<srt::FormFP64>(p, L, st);
             break;
         case SRT_PREC_MIXED:
-            rc = p->bicubic ? launch_batch_dispatch<srt::FormMixedBicubic>(p, st)
-                            : launch_batch_dispatch<srt::FormMixed>(p, st);
+            rc = L.bicubic ? launch_batch_dispatch<srt::FormMixedBicubic>(p, L, st)
+                           : launch_batch_dispatch<srt::FormMixed>(p, L, st);
             break;
         default:
-            rc = launch_batch_dispatch<srt::FormRef>(p, st);
+            rc = launch_batch_dispatch<srt::FormRef>(p, L, st);
             break;
     }
     if (rc != SRT_OK) return rc;
@@ -2047,24 +2075,32 @@ int plan_enqueue_dispatch(srt_plan* p, hipStream_t st) {
     return SRT_OK;
 }
 
+// The plan's launches in order on stream st.
 int plan_enqueue(srt_plan* p, hipStream_t st) {
-    if (p->schedule == SRT_SCHEDULE_DISPATCH) return plan_enqueue_dispatch(p, st);
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, p->d_counter, p->groups);
-    switch (p->precision) {
-        case SRT_PREC_FP64:
-            launch_batch<srt::FormFP64>(p, st);
-            break;
-        case SRT_PREC_MIXED:
-            if (p->bicubic)
-                launch_batch<srt::FormMixedBicubic>(p, st);
-            else
-                launch_batch<srt::FormMixed>(p, st);
-            break;
-        default:
-            launch_batch<srt::FormRef>(p, st);
-            break;
+    for (int li = 0; li < p->n_launches; ++li) {
+        const PlanLaunch& L = p->launches[li];
+        if (p->schedule == SRT_SCHEDULE_DISPATCH) {
+            const int rc = plan_enqueue_dispatch(p, L, st);
+            if (rc != SRT_OK) return rc;
+            continue;
+        }
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(kMaxTileGroups), 0, st, L.d_counter, p->groups);
+        switch (p->precision) {
+            case SRT_PREC_FP64:
+                launch_batch<srt::FormFP64>(p, L, st);
+                break;
+            case SRT_PREC_MIXED:
+                if (L.bicubic)
+                    launch_batch<srt::FormMixedBicubic>(p, L, st);
+                else
+                    launch_batch<srt::FormMixed>(p, L, st);
+                break;
+            default:
+                launch_batch<srt::FormRef>(p, L, st);
+                break;
+        }
+        SRT_HIP(hipGetLastError());
     }
-    SRT_HIP(hipGetLastError());
     return SRT_OK;
 }
 
@@ -2105,10 +2141,14 @@ int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt
         p->total_ray_steps += it.count * (int64_t)it.n_steps;
     }
     p->blocks = 2 * device_cus(device);
-    hipError_t e = hipMalloc(&p->d_inst, sizeof(BatchInst) * n_items);
-    if (e == hipSuccess) e = hipMalloc(&p->d_counter, kMaxTileGroups * sizeof(unsigned long long));
-    if (e == hipSuccess)
-        e = hipMalloc(&p->d_parts, ((size_t)n_items + (size_t)kMaxTileGroups * (n_items + 1)) * sizeof(int64_t));
+    // tables of every launch sized for all items (a layout change can move items between launches)
+    hipError_t e = hipSuccess;
+    for (PlanLaunch& L : p->launches) {
+        if (e == hipSuccess) e = hipMalloc(&L.d_inst, sizeof(BatchInst) * n_items);
+        if (e == hipSuccess) e = hipMalloc(&L.d_counter, kMaxTileGroups * sizeof(unsigned long long));
+        if (e == hipSuccess)
+            e = hipMalloc(&L.d_parts, ((size_t)n_items + (size_t)kMaxTileGroups * (n_items + 1)) * sizeof(int64_t));
+    }
     if (e == hipSuccess) e = hipEventCreate(&p->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&p->ev_stop);
     if (e != hipSuccess) {
@@ -2148,9 +2188,11 @@ int srt_plan_destroy(srt_plan* plan) {
     if (!plan) return SRT_OK;
     (void)hipSetDevice(plan->device);
     plan_drop_graph(plan);
-    if (plan->d_inst) (void)hipFree(plan->d_inst);
-    if (plan->d_counter) (void)hipFree(plan->d_counter);
-    if (plan->d_parts) (void)hipFree(plan->d_parts);
+    for (PlanLaunch& L : plan->launches) {
+        if (L.d_inst) (void)hipFree(L.d_inst);
+        if (L.d_counter) (void)hipFree(L.d_counter);
+        if (L.d_parts) (void)hipFree(L.d_parts);
+    }
     if (plan->order_buf) (void)hipFree(plan->order_buf);
     if (plan->ev_start) (void)hipEventDestroy(plan->ev_start);
     if (plan->ev_stop) (void)hipEventDestroy(plan->ev_stop);
@@ -2301,7 +2343,7 @@ int srt_plan_autotune_brick(srt_plan* plan, void* stream, int* brick) {
     SRT_HIP(hipSetDevice(plan->device));
     int rc = plan_refresh(plan);
     if (rc != SRT_OK) return rc;
-    if (!plan->bicubic)
+    if (plan->n_launches != 1 || !plan->launches[0].bicubic)
         return fail(SRT_ERR_INVALID_ARGUMENT,
                     "srt_plan_autotune_brick: needs a mixed-precision plan whose contexts hold bicubic records");
     std::vector<srt_context*> ctxs;
