@@ -252,10 +252,11 @@ typedef struct srt_plan srt_plan;
  * A plan follows its contexts: after srt_upload or srt_set_layout on one of
  * them, the next launch rebuilds the plan's device table (and its graph) from
  * the contexts' current buffers, so it traces the new snapshot or layout.
- * Each item's output equals its own srt_trace bit for bit.  A SRT_PREC_MIXED
- * plan whose contexts differ in layout (some hold bicubic records, some not)
- * runs two launches on the stream: the record kernel for the items whose
- * contexts hold records, the padded-cube kernel for the others. */
+ * Each item's output equals its own srt_trace bit for bit: the plan runs one
+ * launch per kernel its items need (in a SRT_PREC_MIXED plan, the record
+ * kernel for items whose contexts hold bicubic records and the padded-cube
+ * kernel for the others; for items whose grid edge is a power of two, the
+ * kernels specialised for it). */
 int srt_plan_create(const srt_batch_item* items, int n_items, int precision, srt_plan** plan);
 int srt_plan_destroy(srt_plan* plan);
 

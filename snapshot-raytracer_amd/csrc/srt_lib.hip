@@ -1837,10 +1837,14 @@ int srt_trace_adjoint(srt_context* ctx, int64_t nside, int64_t pix_begin, int64_
 
 // ------------------------------------------------------------ batch plans ----
 // One kernel launch of a plan: the items traced by the same kernel, with that
-// launch's device tables.  A plan has one launch, or two when it is a mixed
-// plan whose contexts differ in layout: the items whose contexts hold bicubic
-// records run the record kernel and the others the padded-cube kernel, each as
-// its own srt_trace would, so every item's maps equal its srt_trace's.
+// launch's device tables.  A plan runs one launch per kernel its items need: in
+// a mixed plan the items whose contexts hold bicubic records run the record
+// kernel and the others the padded-cube kernel, and in every plan the items
+// whose grid edge is a power of two run the kernels specialised for it.  Each
+// item thus runs the kernel of its own srt_trace, and its maps equal that
+// trace's.  48 uneven snapshots (edges 128 to 384) ran 68.1 ms in two launches
+// against 68.6 ms in one on the general kernel (dispatch schedule), and 80.6
+// against 86.7 ms (persistent schedule).
 struct PlanLaunch {
     std::vector<int> items;                   // indices into srt_plan::items
     bool bicubic = false;                     // mixed launch over bicubic records
@@ -1853,7 +1857,7 @@ struct PlanLaunch {
     int64_t total_tiles = 0;      // tiles of the persistent schedule
 };
 
-constexpr int kMaxPlanLaunches = 2;
+constexpr int kMaxPlanLaunches = 4;
 
 struct srt_plan {
     int device = 0;
@@ -1870,7 +1874,7 @@ struct srt_plan {
     std::vector<const uint32_t*> item_map;     // per item: its map in order_buf, or null
     std::vector<uint64_t> item_gen;            // per item: its context's generation when the table was built
     // dispatch by default: at 1024^3 / Nside 2048 / 512 steps a one-item plan takes 3.65 s against 3.77 s
-    // persistent (the single trace launch: 3.66 s), and 48 uneven snapshots 68.6 against 86.7 ms
+    // persistent (the single trace launch: 3.66 s), and 48 uneven snapshots 68.1 against 80.6 ms
     int schedule = SRT_SCHEDULE_DISPATCH;
     int tile_rays = 64;  // one 64-ray group per claim: in-flight rays of an XCD stay as compact as a trace launch's
     int blocks = 0;
@@ -1931,17 +1935,21 @@ int plan_build_table(srt_plan* p) {
     for (int i = 0; i < p->n_inst; ++i)
         if (!p->items[i].ctx->grids)
             return fail(SRT_ERR_NOT_UPLOADED, "plan item " + std::to_string(i) + ": context without snapshot");
-    // the launches: a mixed plan's items whose contexts hold records, then the others
-    std::vector<int> with_rec, rest;
-    for (int i = 0; i < p->n_inst; ++i)
-        (p->precision == SRT_PREC_MIXED && p->items[i].ctx->rec ? with_rec : rest).push_back(i);
+    // the launches: a mixed plan's items whose contexts hold records, then the others, each split into the
+    // items whose grid edge is a power of two and the others
+    std::vector<int> members[2][2];  // [records][power of two]
+    for (int i = 0; i < p->n_inst; ++i) {
+        const int n = p->items[i].ctx->n;
+        members[p->precision == SRT_PREC_MIXED && p->items[i].ctx->rec ? 1 : 0][(n & (n - 1)) == 0 ? 1 : 0].push_back(i);
+    }
     p->n_launches = 0;
-    for (std::vector<int>* members : {&with_rec, &rest})
-        if (!members->empty()) {
-            PlanLaunch& L = p->launches[p->n_launches++];
-            L.items = *members;
-            L.bicubic = members == &with_rec;
-        }
+    for (int r = 1; r >= 0; --r)
+        for (int q = 1; q >= 0; --q)
+            if (!members[r][q].empty()) {
+                PlanLaunch& L = p->launches[p->n_launches++];
+                L.items = members[r][q];
+                L.bicubic = r == 1;
+            }
     SRT_HIP(hipSetDevice(p->device));
     SRT_HIP(hipDeviceSynchronize());
     for (int li = 0; li < p->n_launches; ++li) {
@@ -2343,7 +2351,9 @@ int srt_plan_autotune_brick(srt_plan* plan, void* stream, int* brick) {
     SRT_HIP(hipSetDevice(plan->device));
     int rc = plan_refresh(plan);
     if (rc != SRT_OK) return rc;
-    if (plan->n_launches != 1 || !plan->launches[0].bicubic)
+    bool all_records = plan->n_launches > 0;
+    for (int li = 0; li < plan->n_launches; ++li) all_records = all_records && plan->launches[li].bicubic;
+    if (!all_records)
         return fail(SRT_ERR_INVALID_ARGUMENT,
                     "srt_plan_autotune_brick: needs a mixed-precision plan whose contexts hold bicubic records");
     std::vector<srt_context*> ctxs;
