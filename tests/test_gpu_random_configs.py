@@ -142,8 +142,8 @@ def test_random_plan(srt, fields, case):
     """A random plan: 1-4 snapshots (edges 4-40, bicubic records on some), 1-8 items over them (several items may
     share a snapshot; arbitrary RING ranges, empty ones included; 0 to 3n steps; observers anywhere), FP64, mixed
     or REF, either schedule, LINEAR or NESTED lanes, 1, 3 or 8 tile ranges, launched directly and replayed from a
-    HIP graph: every item equals its own srt_trace bit for bit (a mixed plan whose snapshots differ in layout runs
-    two launches, the record kernel and the padded-cube kernel)."""
+    HIP graph: every item equals its own srt_trace bit for bit (the plan runs one launch per kernel its items need:
+    record or padded-cube kernel by layout, power-of-two or general grid by edge)."""
     import torch
     p = draw_plan(case)
     print(f"[gpu] random plan {case}: {len(p['contexts'])} snapshots {[c['n'] for c in p['contexts']]}, "
