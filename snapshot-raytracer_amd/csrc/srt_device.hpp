@@ -307,22 +307,36 @@ struct Interp {
 
 // Tricubic interpolation as one fused contraction: 16 rows x 3 (x-pass),
 // 16 x 6 (y-pass), 4 x 10 (z-pass) = 328 multiply-adds.  `base` points at
-// tap (-1, -1, -1) of the stencil in the padded grid.  kSliceFence keeps
-// each z-slice's four row loads with its contraction: at 512^3 / Nside 1024 /
-// 256 steps the FP64 kernel then runs 1625 instead of 1722 ms and the REF
-// kernel 929 instead of 947 ms; the mixed padded-grid kernel is 1.5% faster
-// without it.
-template <class R, bool kSliceFence = false, class GT>
+// tap (-1, -1, -1) of the stencil in the padded grid.
+// kGroupedLoads (the kernels with one wavefront per SIMD, FP64 and REF):
+// the 16 row loads are issued together, ahead of a scheduling fence, and each
+// z-slice's contraction stays behind a fence of its own.  With no second
+// wavefront to run meanwhile, the stencil's load latencies then overlap
+// instead of following each other: at 512^3 / Nside 1024 / 256 steps the
+// FP64 kernel runs 1158 instead of 1618 ms (loads grouped per z-slice: 1253
+// ms) and the REF kernel 899 instead of 927 ms.  The mixed padded-grid kernel
+// (two wavefronts per SIMD) gains nothing from either.
+template <class R, bool kGroupedLoads = false, class GT>
 __device__ __forceinline__ Interp<R> tricubic(const GT* __restrict__ base, size_t pitch_y, size_t pitch_z,
                                               const Axis<R>& ax, const Axis<R>& ay, const Axis<R>& az) {
     Interp<R> o;
+    using Row = decltype(load_row(base));
+    Row rows[kGroupedLoads ? 16 : 1];
+    if constexpr (kGroupedLoads) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rows[q] = load_row(base + (q & 3) * pitch_y + (q >> 2) * pitch_z);
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        if constexpr (kSliceFence) __builtin_amdgcn_sched_barrier(0);
+        if constexpr (kGroupedLoads) __builtin_amdgcn_sched_barrier(0);
         R yv, yd, ydd, xd, xdyd, xdd;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const auto q = load_row(base + b * pitch_y + c * pitch_z);
+            Row q;
+            if constexpr (kGroupedLoads)
+                q = rows[4 * c + b];
+            else
+                q = load_row(base + b * pitch_y + c * pitch_z);
             const R p0 = R(q.x), p1 = R(q.y), p2 = R(q.z), p3 = R(q.w);
             const R v = ax.w[0] * p0 + ax.w[1] * p1 + ax.w[2] * p2 + ax.w[3] * p3;
             const R d = ax.d[0] * p0 + ax.d[1] * p1 + ax.d[2] * p2 + ax.d[3] * p3;
@@ -376,16 +390,25 @@ __device__ __forceinline__ Interp<R> tricubic(const GT* __restrict__ base, size_
 // z-pass over the four planes (40) and two doublings: 194 operations, against
 // 328 multiply-adds for tricubic() on the raw taps.  The same interpolant as tricubic(): only the
 // order of operations differs.
+// The 16 record loads are issued together, at raised wavefront priority,
+// ahead of a scheduling fence: at 512^3 / Nside 1024 / 256 steps the record
+// kernel runs 439-441 instead of 450 ms (the priority accounts for ~1 ms).
 template <class R>
 __device__ __forceinline__ Interp<R> bicubic_column(const R* __restrict__ rec, const Powers<R>& px,
                                                     const Powers<R>& py, const Axis<R>& az) {
     R P00[4], P10[4], P20h[4], P01[4], P11[4], P02h[4];
+    decltype(load_row(rec)) rows[16];
+    __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rows[q] = load_row(rec + 4 * q);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         R hv[4], hd[4], hh[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-            const auto c = load_row(rec + 16 * r + 4 * p);
+            const auto c = rows[4 * r + p];
             hv[p] = c.x + py.t * (c.y + py.t * (c.z + py.t * c.w));  // sum_q c[p][q] ty^q
             hd[p] = c.y + py.t2 * c.z + py.t3sq * c.w;                 // d/dty
             hh[p] = c.z + py.t3 * c.w;                                 // (d2/dty2) / 2
@@ -576,10 +599,11 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
             const Axis<RI> wx = cr_axis<RI>(a == 0 ? t1[0] : t0[0]);
             const Axis<RI> wy = cr_axis<RI>(a == 1 ? t1[1] : t0[1]);
             const Axis<RI> wz = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
+            constexpr bool kGrouped = F::kWavesPerSimd == 1;  // FP64 and REF
             if constexpr (F::kGrid64)
-                return tricubic<RI, true>((f == 0 ? G.phi64 : G.b64[a]) + off, py, pz, wx, wy, wz);
+                return tricubic<RI, kGrouped>((f == 0 ? G.phi64 : G.b64[a]) + off, py, pz, wx, wy, wz);
             else
-                return tricubic<RI, F::kRefOffset>((f == 0 ? G.phi : G.b[a]) + off, py, pz, wx, wy, wz);
+                return tricubic<RI, kGrouped>((f == 0 ? G.phi : G.b[a]) + off, py, pz, wx, wy, wz);
         }
     };
 
