@@ -33,15 +33,15 @@ import __graft_entry__  # noqa: E402
 FLOPS_PER_RAY_STEP = {"mixed": 8293.0, "mixed_taps": 12133.0, "fp64": 12793.0}
 # HBM bytes per launch of each kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (the counters scaled by what they read for a 1 GiB device copy), same profile.
-HBM_BYTES_PER_LAUNCH = {"mixed": 28.44e9, "mixed_taps": 108.4e9, "fp64": 7.87e9}
+HBM_BYTES_PER_LAUNCH = {"mixed": 28.46e9, "mixed_taps": 110.2e9, "fp64": 8.75e9}
 # The gather path per launch of each kernel, from rocprofv3 TCP, TA and TCC
 # counters of one launch of this workload (same profile): L1 (TCP) cache-line
 # tag accesses (TCP_TOTAL_CACHE_ACCESSES_sum), L2 read bytes
 # (TCC_READ_SECTORS_sum x 32 B) and the share of cycles the texture-address
 # unit was busy (TA_BUSY_avr over GRBM_GUI_ACTIVE / 8 XCDs).
-GATHER_PER_LAUNCH = {"mixed": {"l1_lines": 2.7518e11, "l2_read_bytes": 6.3734e11, "ta_busy": 0.759},
-                     "mixed_taps": {"l1_lines": 3.2305e11, "l2_read_bytes": 1.38854e12, "ta_busy": 0.725},
-                     "fp64": {"l1_lines": 5.6282e11, "l2_read_bytes": 3.4406e11, "ta_busy": 0.511}}
+GATHER_PER_LAUNCH = {"mixed": {"l1_lines": 2.7518e11, "l2_read_bytes": 6.5088e11, "ta_busy": 0.779},
+                     "mixed_taps": {"l1_lines": 3.2305e11, "l2_read_bytes": 1.38832e12, "ta_busy": 0.724},
+                     "fp64": {"l1_lines": 5.6283e11, "l2_read_bytes": 3.7126e11, "ta_busy": 0.589}}
 # L2 read peak: 8 XCDs x 16 channels (TCC_CYCLE_sum / cycles = 128 channels) x 128 B per channel and clock at
 # 2.4 GHz; the CUs and the clock of the L1 rate
 L2_PEAK_TBPS = 128 * 128 * 2.4e9 / 1e12
