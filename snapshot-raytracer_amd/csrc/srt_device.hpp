@@ -314,7 +314,8 @@ struct Interp {
 // wavefront to run meanwhile, the stencil's load latencies then overlap
 // instead of following each other: at 512^3 / Nside 1024 / 256 steps the
 // FP64 kernel runs 1158 instead of 1618 ms (loads grouped per z-slice: 1253
-// ms) and the REF kernel 899 instead of 927 ms.  The mixed padded-grid kernel
+// ms; 1134 ms with a fence before each field as well) and the REF kernel 899
+// instead of 927 ms.  The mixed padded-grid kernel
 // (two wavefronts per SIMD) gains nothing from either.
 template <class R, bool kGroupedLoads = false, class GT>
 __device__ __forceinline__ Interp<R> tricubic(const GT* __restrict__ base, size_t pitch_y, size_t pitch_z,
@@ -591,9 +592,10 @@ __device__ __forceinline__ void rhs(const Grids& G, const double* __restrict__ x
             const Axis<RI> az = cr_axis<RI>(a == 2 ? t1[2] : t0[2]);
             return bicubic_column<RI>(G.rec[f] + record_offset(G, ix, iy, iz), px, py, az);
         } else {
-            // FP32 taps: the same fence as above (581 instead of 738 ms on the
-            // same workload)
-            if constexpr (!F::kGrid64) __builtin_amdgcn_sched_barrier(0);
+            // the same fence as above: FP32 taps 581 instead of 738 ms on the
+            // same workload, FP64 taps (with their grouped loads) 1134 instead
+            // of 1159 ms
+            __builtin_amdgcn_sched_barrier(0);
             const size_t py = (size_t)G.pad, pz = (size_t)G.pad * (size_t)G.pad;
             const size_t off = (size_t)ix + (size_t)iy * py + (size_t)iz * pz;
             const Axis<RI> wx = cr_axis<RI>(a == 0 ? t1[0] : t0[0]);
